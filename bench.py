@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark: FCHL19 local-kernel KRR energy fit on one MI355X.
+
+Workload (one "step" = one full fit, inputs resident in HBM before the clock starts):
+  20,000 synthetic molecules (seed 1, 5-23 atoms, H/C/N/O/S) -> FCHL19 descriptors
+  (720-dim, FP64) -> element-matched local Gaussian kernel (sigma = 2) written as a
+  20,000 x 20,000 FP64 RFP matrix -> RFP Cholesky with lambda = 1e-8 -> solve for alpha.
+Targets: per-molecule energies minus an element-count baseline (least squares), as the
+reference's LocalKRRModel fits them.  A 2,000-molecule held-out set (the next draws of
+the same stream) gives the test MAE, computed after the timed region.
+
+Prints ONE JSON line (rank 0).  `roofline` is for the dominant kernel (the symmetric
+local kernel), timed with HIP events recorded by libkf_hip on the launch stream;
+`cpu_baseline` is the oracle (oracle/, the reference's algorithm on OpenBLAS/OpenMP)
+on a bounded sample of the same workload, scaled to the full size.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-sample 3000]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("KRR solve s @≤1e-6 rel err vs FP64: plain vs mixed+learned PCG; kernel TFLOP/s")
+PEAK_F64_MFMA_TFLOPS = 78.6
+SIGMA, LAMBDA = 2.0, 1e-8
+ELEMENTS = (1, 6, 7, 8, 16)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def element_baseline(z_list, energies):
+    cnt = np.array([[np.sum(z == e) for e in ELEMENTS] for z in z_list], dtype=np.float64)
+    coef, *_ = np.linalg.lstsq(cnt, energies, rcond=None)
+    return cnt, coef
+
+
+def label_rows(Q, N):
+    ma = Q.shape[1]
+    mask = np.arange(ma)[None, :] < np.clip(N, 0, ma)[:, None]
+    labs, counts = np.unique(Q[mask], return_counts=True)
+    return dict(zip(labs.tolist(), counts.tolist()))
+
+
+def cpu_threads():
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", "")) or len(os.sched_getaffinity(0))
+    except ValueError:
+        return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(C, Q, N, y, sample, full_rows_sq, n_full):
+    """Oracle pipeline on the first `sample` training molecules, scaled to the full fit:
+    descriptors x n, kernel x sum_e R_e^2, Cholesky x n^3."""
+    import oracle
+    from threadpoolctl import threadpool_limits
+
+    cores = cpu_threads()
+    with threadpool_limits(limits=cores):
+        Cs, Qs, Ns, ys = C[:sample], Q[:sample], N[:sample], y[:sample]
+        oracle.fchl19_batch(Cs[:64], Qs[:64], Ns[:64])  # warm-up (thread pool spin-up)
+        t0 = time.perf_counter()
+        Xs = oracle.fchl19_batch(Cs, Qs, Ns)
+        t1 = time.perf_counter()
+        arf = oracle.kernel_symm_rfp(Xs, Qs, Ns, SIGMA)
+        t2 = time.perf_counter()
+        oracle.cho_solve_rfp(arf, ys, LAMBDA)
+        t3 = time.perf_counter()
+    rs_sample = sum(v * v for v in label_rows(Qs, Ns).values())
+    s_desc = (t1 - t0) * n_full / sample
+    s_kern = (t2 - t1) * full_rows_sq / rs_sample
+    s_chol = (t3 - t2) * (n_full / sample) ** 3
+    return {
+        "value": round(s_desc + s_kern + s_chol, 3),
+        "unit": "s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle (C/OpenMP + scipy OpenBLAS) on the first {sample} of the 20,000 "
+                   f"training molecules: descriptors {t1 - t0:.2f} s, FP64 symmetric RFP "
+                   f"kernel {t2 - t1:.2f} s, dpftrf+dpftrs {t3 - t2:.3f} s; scaled to 20,000 "
+                   f"by n (descriptors), sum_e R_e^2 (kernel) and n^3 (Cholesky): "
+                   f"{s_desc:.1f} + {s_kern:.1f} + {s_chol:.1f} s"),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-train", type=int, default=20000)
+    ap.add_argument("--n-test", type=int, default=2000)
+    ap.add_argument("--cpu-sample", type=int, default=3000,
+                    help="molecules in the CPU-baseline sample (0 skips it)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path would shard by molecule-pair tiles, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from kernelforge_amd import capi, synthetic
+
+    L = capi.lib()
+    dev = torch.device("cuda:0")
+    name = ctypes_name(L)
+    log(f"device {name}")
+
+    ntr, nte = args.n_train, args.n_test
+    t0 = time.perf_counter()
+    coords, zs, energies = synthetic.random_molecules(ntr + nte, 1)
+    C, Q, N = synthetic.pad_batch(coords, zs)
+    cnt, coef = element_baseline(zs[:ntr], energies[:ntr])
+    y = energies[:ntr] - cnt @ coef
+    log(f"generated {ntr + nte} molecules in {time.perf_counter() - t0:.1f} s")
+    ma = Q.shape[1]
+    rs = capi.rep_size()
+
+    Ctr = torch.from_numpy(C[:ntr]).to(dev)
+    Qtr = torch.from_numpy(Q[:ntr]).to(dev)
+    Ntr = torch.from_numpy(N[:ntr]).to(dev)
+    ytr = torch.from_numpy(y).to(dev)
+    Xtr = torch.empty((ntr, ma, rs), dtype=torch.float64, device=dev)
+    arf = torch.empty(ntr * (ntr + 1) // 2, dtype=torch.float64, device=dev)
+    alpha = torch.empty(ntr, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        capi.fchl19(Ctr.data_ptr(), Qtr.data_ptr(), Ntr.data_ptr(), ntr, ma, Xtr.data_ptr(), 0,
+                    stream)
+        capi.local_kernel_symm_rfp(capi.KF_PREC_F64, Xtr.data_ptr(), Qtr.data_ptr(),
+                                   Ntr.data_ptr(), ntr, ma, rs, SIGMA, arf.data_ptr(), stream)
+        capi.rfp_potrf(capi.KF_PREC_F64, arf.data_ptr(), ntr, LAMBDA, stream)
+        alpha.copy_(ytr)
+        capi.rfp_potrs(capi.KF_PREC_F64, arf.data_ptr(), ntr, alpha.data_ptr(), 1, stream)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup {w + 1}/{args.warmup} done")
+    L.kf_timer_reset()
+    L.kf_timer_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        log(f"step {s + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    L.kf_timer_enable(0)
+    sec_per_fit = elapsed / args.steps
+    timers = {k: capi.timer_read(v) for k, v in
+              (("fchl19", capi.TM_FCHL19), ("kernel", capi.TM_LOCAL_SYMM),
+               ("potrf", capi.TM_POTRF), ("potrs", capi.TM_POTRS))}
+    avg_ms = {k: (ms / c if c else None) for k, (ms, c) in timers.items()}
+
+    rows = label_rows(Q[:ntr], N[:ntr])
+    rows_sq = sum(v * v for v in rows.values())
+    kernel_flop = rs * rows_sq  # symmetric half of 2 * R_e^2 * rep, summed over labels
+    achieved = kernel_flop / (avg_ms["kernel"] * 1e-3) / 1e12
+
+    # test MAE (after the timed region; not part of the step)
+    Xte = torch.empty((nte, ma, rs), dtype=torch.float64, device=dev)
+    Cte = torch.from_numpy(C[ntr:]).to(dev)
+    Qte = torch.from_numpy(Q[ntr:]).to(dev)
+    Nte = torch.from_numpy(N[ntr:]).to(dev)
+    capi.fchl19(Cte.data_ptr(), Qte.data_ptr(), Nte.data_ptr(), nte, ma, Xte.data_ptr(), 0, stream)
+    Kte = torch.empty((nte, ntr), dtype=torch.float64, device=dev)
+    capi.local_kernel(capi.KF_PREC_F64, Xte.data_ptr(), Xtr.data_ptr(), Qte.data_ptr(),
+                      Qtr.data_ptr(), Nte.data_ptr(), Ntr.data_ptr(), nte, ntr, ma, ma, rs, SIGMA,
+                      Kte.data_ptr(), stream)
+    torch.cuda.synchronize()
+    cnt_te = np.array([[np.sum(z == e) for e in ELEMENTS] for z in zs[ntr:]], dtype=np.float64)
+    pred = Kte.cpu().numpy() @ alpha.cpu().numpy() + cnt_te @ coef
+    mae = float(np.mean(np.abs(pred - energies[ntr:])))
+    log(f"fit {sec_per_fit:.3f} s/step, kernel {avg_ms['kernel']:.1f} ms ({achieved:.1f} TFLOP/s), "
+        f"potrf {avg_ms['potrf']:.1f} ms, potrs {avg_ms['potrs']:.1f} ms, test MAE {mae:.4f} Ha")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"CPU baseline on {args.cpu_sample} molecules ...")
+        cpu = cpu_baseline(C[:ntr], Q[:ntr], N[:ntr], y, min(args.cpu_sample, ntr), rows_sq, ntr)
+        log(f"CPU baseline {cpu['value']} s (scaled)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(sec_per_fit, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_per_fit * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"FCHL19 local-kernel KRR energy fit, plain FP64: {ntr} synthetic "
+                         f"molecules (seed 1), descriptors + symmetric RFP kernel + RFP "
+                         f"Cholesky solve"),
+            "n_train": ntr, "n_test": nte, "max_atoms": ma, "rep": rs, "sigma": SIGMA,
+            "lambda": LAMBDA, "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_F64_MFMA_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_F64_MFMA_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "local_scalar_f64_kernel<SYMM_RFP>",
+            "flop_per_launch": kernel_flop,
+            "avg_launch_ms": round(avg_ms["kernel"], 3),
+        },
+        "cpu_baseline": cpu,
+        "breakdown_ms": {k: (round(v, 3) if v is not None else None) for k, v in avg_ms.items()},
+        "test_mae_ha": round(mae, 5),
+        "device": name,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def ctypes_name(L):
+    import ctypes
+
+    buf = ctypes.create_string_buffer(256)
+    L.kf_device_name(buf, 256)
+    return buf.value.decode(errors="replace")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,144 @@
+/*
+ * kf_hip.h -- C ABI of libkf_hip.so, the MI355X (gfx950) implementation of
+ * kernelforge's FCHL19 local-kernel KRR hot path.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer (HBM), except the small parameter
+ *    arrays marked "host" (element lists, basis grids).  Layouts are the
+ *    reference's row-major ones:
+ *        X   (nm, max_atoms, rep)                  float64
+ *        dX  (nm, max_atoms, rep, 3*max_atoms)     float64
+ *        Q   (nm, max_atoms) int32 labels, N (nm,) int32 atom counts
+ *        RFP arrays: LAPACK Rectangular Full Packed, TRANSR='N', UPLO='U',
+ *        length n(n+1)/2 (reference src/rfp_utils.hpp:14-22).
+ *    Atom counts are clamped to [0, max_atoms] as the reference does
+ *    (src/local_kernels.cpp:40).  naq = 3 * sum(clamped N).
+ *  - `stream` is a hipStream_t (NULL = the null stream).  Calls are
+ *    stream-ordered; entry points that need host-side metadata (labels, atom
+ *    counts) copy it back and synchronise the stream.  Calls on one device must
+ *    not overlap: they share the library's device workspace.
+ *  - Return value: KF_OK (0) on success, a negative KF_ERR_* code for invalid
+ *    arguments or runtime failures (kf_last_error() has the message), or a
+ *    positive factorisation `info` (leading minor that is not positive
+ *    definite, 1-based) from the Cholesky entry points.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * andersx/kernelforge).
+ */
+#ifndef KF_HIP_H
+#define KF_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KF_OK 0
+#define KF_ERR_ARG (-1)     /* bad shape / size / pointer                 -> ValueError   */
+#define KF_ERR_SIGMA (-2)   /* sigma <= 0 or not finite                    -> ValueError   */
+#define KF_ERR_ELEMENT (-3) /* nuclear charge not in `elements`            -> RuntimeError */
+#define KF_ERR_BASIS (-4)   /* Rs2 <= 0, odd nabasis, unsupported sizes    -> ValueError   */
+#define KF_ERR_HIP (-5)     /* HIP runtime failure                         -> RuntimeError */
+#define KF_ERR_NOMEM (-6)   /* device allocation failed                    -> MemoryError  */
+#define KF_ERR_NOCONV (-7)  /* iterative solver did not converge           -> RuntimeError */
+
+/* Arithmetic of a kernel / factorisation. */
+#define KF_PREC_F64 0    /* FP64 throughout (the reference's precision)                   */
+#define KF_PREC_BF16X3 1 /* BF16x3 split MFMA contraction, FP32 accumulate/exp, FP32 out  */
+#define KF_PREC_F32 2    /* FP32 (factorisations)                                         */
+
+/* Library / device information. */
+int kf_version(void);
+const char *kf_last_error(void);
+int kf_device_name(char *buf, int len);
+
+/* Kernel timers (instrumentation; no reference counterpart -- the reference prints
+ * ad-hoc phase timers, src/local_kernels.cpp:2564-2568).  When enabled, entry
+ * points bracket their main kernel launch(es) with HIP events on the launch
+ * stream; kf_timer_read synchronises on them and returns the summed milliseconds
+ * and the number of bracketed calls for one timer id:
+ *   0 fchl19, 1 local symmetric kernel, 2 local rectangular kernel,
+ *   3 RFP potrf, 4 RFP potrs, 5 E+F kernels, 6 matvecs. */
+int kf_timer_enable(int on);
+int kf_timer_reset(void);
+int kf_timer_read(int id, double *total_ms, int *count);
+
+/* ------------------------------------------------------------------------ */
+/* FCHL19 representation                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Representation (and optionally its coordinate Jacobian) of a padded batch of
+ * molecules.  Replaces kf::fchl19::generate_fchl_acsf (src/fchl19_repr.cpp:89)
+ * and generate_fchl_acsf_and_gradients (src/fchl19_repr.cpp:308), called per
+ * molecule by models/representations.py:107-145.
+ *   coords (nm, max_atoms, 3), Q (nm, max_atoms) nuclear charges, N (nm,)
+ *   elements/Rs2/Rs3 are HOST arrays; w3 = sqrt(eta3/pi) * three_body_weight
+ *   (src/fchl19_repr_bindings.cpp:87).
+ *   X  (nm, max_atoms, rep) out, rows >= N[m] zeroed.
+ *   dX (nm, max_atoms, rep, 3*max_atoms) out, or NULL for representation only.
+ *   With dX != NULL the three-body angular term keeps only the first harmonic,
+ *   as the reference's gradient path does (src/fchl19_repr.cpp:449,523-528). */
+int kf_fchl19_f64(const double *coords, const int *Q, const int *N, int nm, int max_atoms,
+                  const int *elements, int nelem, const double *Rs2, int nRs2, const double *Rs3,
+                  int nRs3, int nabasis, double eta2, double eta3, double zeta, double rcut,
+                  double acut, double two_body_decay, double three_body_decay, double w3,
+                  double *X, double *dX, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Local (element-matched) Gaussian kernels                                 */
+/*   K[a,b] = sum_{i in a, j in b, Q_i == Q_j} exp(-|x_i - x_j|^2 / 2 sigma^2) */
+/* ------------------------------------------------------------------------ */
+
+/* Symmetric kernel written straight into RFP (upper).  Replaces
+ * kf::fchl19::kernel_gaussian_symm_rfp (src/local_kernels.cpp:403).
+ * precision KF_PREC_F64: arf is double*; KF_PREC_BF16X3: arf is float*. */
+int kf_local_kernel_symm_rfp(int precision, const double *X, const int *Q, const int *N, int nm,
+                             int max_atoms, int rep, double sigma, void *arf, void *stream);
+
+/* Symmetric kernel, dense (nm, nm) row-major, both triangles.  Replaces
+ * kf::fchl19::kernel_gaussian_symm (src/local_kernels.cpp:584).  K is double*
+ * (KF_PREC_F64) or float* (KF_PREC_BF16X3). */
+int kf_local_kernel_symm(int precision, const double *X, const int *Q, const int *N, int nm,
+                         int max_atoms, int rep, double sigma, void *K, void *stream);
+
+/* Rectangular kernel (nm1, nm2) row-major.  Replaces kf::fchl19::kernel_gaussian
+ * (src/local_kernels.cpp:155).  K is double* (F64) or float* (BF16X3). */
+int kf_local_kernel(int precision, const double *X1, const double *X2, const int *Q1,
+                    const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
+                    int max_atoms1, int max_atoms2, int rep, double sigma, void *K, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* kernelmath: RFP Cholesky                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* In-place Cholesky of an RFP (TRANSR='N', UPLO='U') matrix after adding l2 to
+ * its diagonal; on return arf holds the factor exactly where LAPACK dpftrf
+ * leaves it.  Replaces the dpftrf_ call of kf::math::solve_cholesky_rfp
+ * (src/math.cpp:156-165).  precision KF_PREC_F64 (double*) or KF_PREC_F32
+ * (float*).  Returns 0 or the positive LAPACK-style info. */
+int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *stream);
+
+/* Solve (L L^T) x = b in place for nrhs right-hand sides stored as columns of b
+ * (b is (nrhs, n) row-major, i.e. each RHS contiguous) with the factor from
+ * kf_rfp_potrf.  Replaces the dpftrs_ call (src/math.cpp:167-170).
+ * precision KF_PREC_F64 (double arf, double b) or KF_PREC_F32 (float arf,
+ * double b: the solve accumulates in FP64). */
+int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream);
+
+/* Full (column-major, leading dimension lda) -> RFP, LAPACK dtrttf semantics for
+ * all TRANSR in {'N','T'} x UPLO in {'U','L'}.  Replaces the dtrttf_ call of
+ * kf::math::full_to_rfp (src/math.cpp:173-179); kernelmath.full_to_rfp passes the
+ * C-order buffer with UPLO swapped exactly as src/math_bindings.cpp:89-120 does. */
+int kf_trttf(char transr, char uplo, int n, const double *A, int lda, double *arf, void *stream);
+
+/* RFP -> full (column-major), LAPACK dtfttr semantics; with symmetrize != 0 the
+ * other triangle is mirrored as kf::math::rfp_to_full does (src/math.cpp:181-211). */
+int kf_tfttr(char transr, char uplo, int n, const double *arf, double *A, int lda, int symmetrize,
+             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KF_HIP_H */

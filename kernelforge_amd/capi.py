@@ -1,0 +1,133 @@
+"""ctypes binding of the C ABI in ``include/kf_hip.h`` (``libkf_hip.so``).
+
+This is the binding a Python caller holding device memory uses (e.g. torch tensors
+on ``cuda:0``): every array argument is a device pointer passed as an integer
+(``tensor.data_ptr()``), ``stream`` is a ``hipStream_t`` (``torch.cuda.current_stream()
+.cuda_stream``) or 0.  Status codes are turned into the exceptions the reference
+raises (ValueError / RuntimeError / MemoryError).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KF_OK = 0
+KF_ERR_ARG, KF_ERR_SIGMA, KF_ERR_ELEMENT, KF_ERR_BASIS = -1, -2, -3, -4
+KF_ERR_HIP, KF_ERR_NOMEM, KF_ERR_NOCONV = -5, -6, -7
+KF_PREC_F64, KF_PREC_BF16X3, KF_PREC_F32 = 0, 1, 2
+
+_c_int, _c_double, _c_char, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "kf_version": [],
+    "kf_last_error": [],
+    "kf_device_name": [ctypes.c_char_p, _c_int],
+    "kf_timer_enable": [_c_int],
+    "kf_timer_reset": [],
+    "kf_timer_read": [_c_int, ctypes.POINTER(_c_double), ctypes.POINTER(_c_int)],
+    "kf_fchl19_f64": [_vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int,
+                      _c_int, _c_double, _c_double, _c_double, _c_double, _c_double, _c_double,
+                      _c_double, _c_double, _vp, _vp, _vp],
+    "kf_local_kernel_symm_rfp": [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_double, _vp,
+                                 _vp],
+    "kf_local_kernel_symm": [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp],
+    "kf_local_kernel": [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int,
+                        _c_int, _c_double, _vp, _vp],
+    "kf_rfp_potrf": [_c_int, _vp, _c_int, _c_double, _vp],
+    "kf_rfp_potrs": [_c_int, _vp, _c_int, _vp, _c_int, _vp],
+    "kf_trttf": [_c_char, _c_char, _c_int, _vp, _c_int, _vp, _vp],
+    "kf_tfttr": [_c_char, _c_char, _c_int, _vp, _vp, _c_int, _c_int, _vp],
+}
+_RESTYPE = {"kf_last_error": ctypes.c_char_p}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libkf_hip.so (raises OSError when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        L = ctypes.CDLL(os.path.join(_HERE, "libkf_hip.so"))
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, _c_int)
+        _LIB = L
+    return _LIB
+
+
+TM_FCHL19, TM_LOCAL_SYMM, TM_LOCAL_RECT, TM_POTRF, TM_POTRS, TM_FULL, TM_MATVEC = range(7)
+
+
+def timer_read(tid: int):
+    """(total milliseconds, number of bracketed calls) of one kernel timer."""
+    ms, cnt = _c_double(0.0), _c_int(0)
+    check(lib().kf_timer_read(tid, ctypes.byref(ms), ctypes.byref(cnt)), "kf_timer_read")
+    return ms.value, cnt.value
+
+
+def check(rc: int, what: str) -> int:
+    if rc == KF_OK:
+        return rc
+    msg = f"{what}: {lib().kf_last_error().decode(errors='replace')}"
+    if rc in (KF_ERR_ARG, KF_ERR_SIGMA, KF_ERR_BASIS):
+        raise ValueError(msg)
+    if rc == KF_ERR_NOMEM:
+        raise MemoryError(msg)
+    if rc > 0:
+        raise RuntimeError(f"{what} failed, info={rc}")
+    raise RuntimeError(msg)
+
+
+def fchl_basis(elements=(1, 6, 7, 8, 16), nRs2=24, nRs3=20, nFourier=1, rcut=8.0, acut=8.0,
+               eta3=2.7, three_body_weight=13.4, grad=False):
+    """Grids as the reference bindings build them (src/fchl19_repr_bindings.cpp:72-87)."""
+    Rs2 = [rcut * i / nRs2 for i in range(1, nRs2 + 1)]
+    Rs3 = [acut * i / nRs3 for i in range(1, nRs3 + 1)]
+    nab = 2 * nFourier
+    if grad:
+        nab = max(nab, 2)
+    w3 = math.sqrt(eta3 / math.pi) * three_body_weight
+    return list(elements), Rs2, Rs3, nab, w3
+
+
+def fchl19(coords, Q, N, nm, max_atoms, X, dX=0, stream=0, elements=(1, 6, 7, 8, 16), nRs2=24,
+           nRs3=20, nFourier=1, eta2=0.32, eta3=2.7, zeta=math.pi, rcut=8.0, acut=8.0,
+           two_body_decay=1.8, three_body_decay=0.57, three_body_weight=13.4):
+    """kf_fchl19_f64 on device pointers (dX=0 for the representation only)."""
+    el, Rs2, Rs3, nab, w3 = fchl_basis(elements, nRs2, nRs3, nFourier, rcut, acut, eta3,
+                                       three_body_weight, grad=bool(dX))
+    el_a = (ctypes.c_int * len(el))(*el)
+    r2 = (ctypes.c_double * len(Rs2))(*Rs2)
+    r3 = (ctypes.c_double * len(Rs3))(*Rs3)
+    rc = lib().kf_fchl19_f64(coords, Q, N, nm, max_atoms, ctypes.cast(el_a, _vp), len(el),
+                             ctypes.cast(r2, _vp), len(Rs2), ctypes.cast(r3, _vp), len(Rs3), nab,
+                             eta2, eta3, zeta, rcut, acut, two_body_decay, three_body_decay, w3,
+                             X, dX or None, stream or None)
+    return check(rc, "kf_fchl19_f64")
+
+
+def rep_size(nelem=5, nRs2=24, nRs3=20, nabasis=2):
+    return nelem * nRs2 + nelem * (nelem + 1) // 2 * nRs3 * nabasis
+
+
+def local_kernel_symm_rfp(precision, X, Q, N, nm, max_atoms, rep, sigma, arf, stream=0):
+    return check(lib().kf_local_kernel_symm_rfp(precision, X, Q, N, nm, max_atoms, rep, sigma,
+                                                arf, stream or None), "kf_local_kernel_symm_rfp")
+
+
+def local_kernel(precision, X1, X2, Q1, Q2, N1, N2, nm1, nm2, ma1, ma2, rep, sigma, K, stream=0):
+    return check(lib().kf_local_kernel(precision, X1, X2, Q1, Q2, N1, N2, nm1, nm2, ma1, ma2, rep,
+                                       sigma, K, stream or None), "kf_local_kernel")
+
+
+def rfp_potrf(precision, arf, n, l2, stream=0):
+    return check(lib().kf_rfp_potrf(precision, arf, n, l2, stream or None), "DPFTRF")
+
+
+def rfp_potrs(precision, arf, n, b, nrhs=1, stream=0):
+    return check(lib().kf_rfp_potrs(precision, arf, n, b, nrhs, stream or None), "DPFTRS")

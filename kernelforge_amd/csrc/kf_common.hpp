@@ -1,0 +1,105 @@
+// kf_common.hpp -- shared host/device helpers for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/kf_hip.h"
+
+namespace kf {
+
+// ---------------------------------------------------------------------------
+// Error plumbing: every C-ABI entry point returns an int status.  HIP failures
+// map to KF_ERR_HIP and the message is kept for kf_last_error().
+// ---------------------------------------------------------------------------
+void set_last_error(const char *fmt, ...);
+
+#define KF_HIP_TRY(expr)                                                              \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            ::kf::set_last_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr,         \
+                                 hipGetErrorString(_e));                              \
+            return KF_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define KF_REQUIRE(cond, code, ...)                                                   \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            ::kf::set_last_error(__VA_ARGS__);                                        \
+            return (code);                                                            \
+        }                                                                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Grow-only device workspace, one set of slots per device.  Slots are named by
+// small integers so independent scratch buffers of one call do not alias.
+// Calls on one device are serialised by the caller's stream; the cache itself is
+// guarded by a mutex for the host-side bookkeeping.
+// ---------------------------------------------------------------------------
+enum WsSlot {
+    WS_PACK = 0,   // packed per-label descriptor rows
+    WS_PACK2,      // second set (rectangular kernels)
+    WS_PLAN,       // plan tables (row offsets, source indices, tiles)
+    WS_PLAN2,
+    WS_AUX0,
+    WS_AUX1,
+    WS_AUX2,
+    WS_AUX3,
+    WS_NSLOTS
+};
+void *workspace(int slot, size_t bytes, hipStream_t stream);
+
+// Pinned host staging for small metadata round trips.
+void *host_staging(int slot, size_t bytes);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Kernel timers: when enabled (kf_timer_enable), entry points bracket their main
+// kernel with HIP events recorded on the launch stream; kf_timer_read sums them.
+enum TimerId {
+    TM_FCHL19 = 0,
+    TM_LOCAL_SYMM,
+    TM_LOCAL_RECT,
+    TM_POTRF,
+    TM_POTRS,
+    TM_FULL,
+    TM_MATVEC,
+    TM_COUNT
+};
+struct ScopedTimer {
+    int id;
+    hipStream_t stream;
+    void *handle;
+    ScopedTimer(int id_, hipStream_t s);
+    void stop();  // record the end event now (idempotent)
+    ~ScopedTimer() { stop(); }
+};
+
+// ---------------------------------------------------------------------------
+// RFP addressing, TRANSR='N', UPLO='U' (the layout LAPACK dpftrf('N','U') and
+// the reference kernels use).  Element (r, c) of the LOWER triangle of the full
+// symmetric matrix (r >= c) -- equivalently the upper element (c, r) -- lives at
+//   r <  k : (k + 1 + r) + c * s      (column-major block, leading dim s)
+//   r >= k : (r - k) * s + c          (row-major block, leading dim s)
+// with k = n / 2 and s = n + 1 (n even) or n (n odd).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline size_t rfp_lower(size_t n, size_t r, size_t c) {
+    const size_t k = n >> 1;
+    const size_t s = (n & 1) ? n : n + 1;
+    return (r < k) ? (k + 1 + r) + c * s : (r - k) * s + c;
+}
+// Upper element (i, j), i <= j.
+__host__ __device__ inline size_t rfp_upper(size_t n, size_t i, size_t j) {
+    return rfp_lower(n, j, i);
+}
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace kf

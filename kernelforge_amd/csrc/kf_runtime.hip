@@ -1,0 +1,151 @@
+// kf_runtime.hip -- error state, device workspace cache, library info.
+#include <cstdarg>
+#include <cstring>
+#include <string>
+
+#include "kf_common.hpp"
+
+namespace kf {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+namespace {
+constexpr int kMaxDevices = 16;
+struct Slot {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+Slot g_ws[kMaxDevices][WS_NSLOTS];
+Slot g_host[WS_NSLOTS];
+}  // namespace
+
+// Grow-only: a slot is reallocated (after draining the stream) only when a call
+// needs more than it holds.  Sizes are rounded up to 2 MiB so that slowly
+// growing requests do not reallocate every call.
+void *workspace(int slot, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) bytes = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Slot &s = g_ws[dev][slot];
+    if (s.bytes >= bytes) return s.ptr;
+    if (s.ptr) {
+        if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+        (void)hipFree(s.ptr);
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    const size_t want = (bytes + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    s.ptr = p;
+    s.bytes = want;
+    return p;
+}
+
+void *host_staging(int slot, size_t bytes) {
+    if (bytes == 0) bytes = 64;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Slot &s = g_host[slot];
+    if (s.bytes >= bytes) return s.ptr;
+    if (s.ptr) (void)hipHostFree(s.ptr);
+    s.ptr = nullptr;
+    s.bytes = 0;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    s.ptr = p;
+    s.bytes = bytes;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timers
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tm_mutex;
+bool g_tm_enabled = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_tm_events[TM_COUNT];
+}  // namespace
+
+ScopedTimer::ScopedTimer(int id_, hipStream_t s) : id(id_), stream(s), handle(nullptr) {
+    std::lock_guard<std::mutex> lock(g_tm_mutex);
+    if (!g_tm_enabled) return;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+    (void)hipEventRecord(a, stream);
+    g_tm_events[id].push_back({a, b});
+    handle = reinterpret_cast<void *>(b);
+}
+void ScopedTimer::stop() {
+    if (handle) (void)hipEventRecord(reinterpret_cast<hipEvent_t>(handle), stream);
+    handle = nullptr;
+}
+
+}  // namespace kf
+
+extern "C" int kf_timer_enable(int on) {
+    std::lock_guard<std::mutex> lock(kf::g_tm_mutex);
+    kf::g_tm_enabled = on != 0;
+    return KF_OK;
+}
+
+extern "C" int kf_timer_reset(void) {
+    std::lock_guard<std::mutex> lock(kf::g_tm_mutex);
+    for (auto &v : kf::g_tm_events) {
+        for (auto &e : v) {
+            (void)hipEventSynchronize(e.second);
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        v.clear();
+    }
+    return KF_OK;
+}
+
+extern "C" int kf_timer_read(int id, double *total_ms, int *count) {
+    KF_REQUIRE(id >= 0 && id < kf::TM_COUNT && total_ms && count, KF_ERR_ARG, "bad timer id");
+    std::lock_guard<std::mutex> lock(kf::g_tm_mutex);
+    double tot = 0.0;
+    for (auto &e : kf::g_tm_events[id]) {
+        KF_HIP_TRY(hipEventSynchronize(e.second));
+        float ms = 0.f;
+        KF_HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *count = (int)kf::g_tm_events[id].size();
+    return KF_OK;
+}
+
+extern "C" int kf_version(void) { return 1; }
+
+extern "C" const char *kf_last_error(void) { return kf::g_last_error.c_str(); }
+
+extern "C" int kf_device_name(char *buf, int len) {
+    int dev = 0;
+    KF_HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    KF_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (buf && len > 0) {
+        std::snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+                      prop.multiProcessorCount);
+    }
+    return KF_OK;
+}

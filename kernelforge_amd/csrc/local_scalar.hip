@@ -1,0 +1,390 @@
+// local_scalar.hip -- element-matched local Gaussian kernel (scalar forms) on gfx950.
+//
+//   K[a,b] = sum_{i in a, j in b, Q_i == Q_j} exp(-(|x_i|^2 + |x_j|^2 - 2 x_i.x_j) / 2 sigma^2)
+//
+// Reference: kernel_gaussian (src/local_kernels.cpp:155-298), kernel_gaussian_symm_rfp
+// (:403-577), kernel_gaussian_symm (:584-758).  The reference packs rows per label,
+// runs DSYRK/DGEMM (alpha=-2) on 8192-row tiles and scatters an OpenMP exp-sum.
+//
+// gfx950 design
+//  * Atoms are packed by (label, molecule) once (local_plan.hpp), so the rows of a
+//    molecule block for one label are one contiguous range.
+//  * The OUTPUT is tiled: one workgroup owns a 32x32 block of molecule pairs (upper
+//    triangle of blocks for the symmetric forms).  For every label it runs the
+//    rows(A-block) x rows(B-block) Gram product as 64x64 sub-tiles on FP64 MFMA
+//    (v_mfma_f64_16x16x4, 2x2 waves of 32x32), with operands staged through LDS
+//    and the next k-chunk prefetched into registers during the MFMAs.
+//  * Epilogue fused per sub-tile: squared norms (FP64), exp, then a deterministic
+//    two-stage segmented reduction in LDS (rows -> molecule columns, then molecule
+//    rows) into the workgroup's 32x32 accumulator.  No atomics; each K entry is
+//    written exactly once, straight into RFP (UPLO='U') or dense layout.
+#include <cmath>
+
+#include "kf_common.hpp"
+#include "local_plan.hpp"
+
+namespace kf {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int MB = 32;       // molecules per output block
+constexpr int TS = 64;       // sub-tile edge (packed rows)
+constexpr int KC = 16;       // k chunk staged in LDS
+constexpr int LDA = KC + 1;  // LDS pitch of operand tiles (doubles)
+constexpr int EP = TS + 1;   // LDS pitch of the exp tile
+constexpr int NT = 256;      // threads per workgroup
+
+enum Mode { SYMM_RFP = 0, SYMM_DENSE = 1, RECT = 2 };
+
+// pack: Xp[row, k] = X[src[row], k] (k < rep), 0 for rep <= k < rep_pad;
+// n2[row] = |x|^2 in FP64.  One wave per row.
+__global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ X,
+                                                     const int *__restrict__ src, int rows,
+                                                     int rep, int rep_pad,
+                                                     double *__restrict__ Xp,
+                                                     double *__restrict__ n2) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= rows) return;
+    const double *s = X + (size_t)src[row] * rep;
+    double *d = Xp + (size_t)row * rep_pad;
+    double acc = 0.0;
+    for (int k = lane; k < rep_pad; k += 64) {
+        const double v = k < rep ? s[k] : 0.0;
+        d[k] = v;
+        acc += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) n2[row] = acc;
+}
+
+__device__ inline void decode_upper(int t, int nb, int &A, int &B) {
+    // t enumerates (A, B), A <= B, row by row: offset(A) = A*nb - A(A-1)/2
+    double dn = 2.0 * nb + 1.0;
+    int a = (int)floor((dn - sqrt(dn * dn - 8.0 * t)) * 0.5);
+    if (a < 0) a = 0;
+    while (a > 0 && (long)a * nb - (long)a * (a - 1) / 2 > t) --a;
+    while ((long)(a + 1) * nb - (long)(a + 1) * a / 2 <= t) ++a;
+    A = a;
+    B = a + (int)(t - ((long)a * nb - (long)a * (a - 1) / 2));
+}
+
+template <int MODE>
+__global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
+    const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
+    const double *__restrict__ Xp2, const double *__restrict__ n2b, const int *__restrict__ mr2,
+    int L, int nm1, int nm2, int nblk1, int nblk2, int rep_pad, double cexp,
+    double *__restrict__ out) {
+    __shared__ double As[TS * LDA];
+    __shared__ double Bs[TS * LDA];
+    __shared__ double Es[TS * EP];
+    __shared__ double part[TS * MB];
+    __shared__ double acc[MB * MB];
+    __shared__ int rowmol[TS], colmol[TS];
+    __shared__ int mrA[MB + 1], mrB[MB + 1];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    int A, B;
+    if (MODE == RECT) {
+        A = blockIdx.x / nblk2;
+        B = blockIdx.x % nblk2;
+    } else {
+        decode_upper(blockIdx.x, nblk1, A, B);
+    }
+    const int a0 = A * MB, na = min(MB, nm1 - a0);
+    const int b0 = B * MB, nb = min(MB, nm2 - b0);
+    const bool diag = (MODE != RECT) && (A == B);
+
+    for (int t = tid; t < MB * MB; t += NT) acc[t] = 0.0;
+
+    for (int l = 0; l < L; ++l) {
+        __syncthreads();
+        if (tid <= na) mrA[tid] = mr1[(size_t)l * (nm1 + 1) + a0 + tid];
+        if (tid >= 64 && tid - 64 <= nb) mrB[tid - 64] = mr2[(size_t)l * (nm2 + 1) + b0 + tid - 64];
+        __syncthreads();
+        const int ra0 = mrA[0], ra1 = mrA[na], rb0 = mrB[0], rb1 = mrB[nb];
+        if (ra1 == ra0 || rb1 == rb0) continue;
+        const int nsr = (ra1 - ra0 + TS - 1) / TS, nsc = (rb1 - rb0 + TS - 1) / TS;
+        for (int sr = 0; sr < nsr; ++sr) {
+            for (int sc = diag ? sr : 0; sc < nsc; ++sc) {
+                const int r0 = ra0 + sr * TS, rn = min(TS, ra1 - r0);
+                const int c0 = rb0 + sc * TS, cn = min(TS, rb1 - c0);
+                __syncthreads();  // previous sub-tile's reduction is done with part/rowmol
+                // molecule of every row / column of the sub-tile
+                if (tid < TS) {
+                    int m = -1;
+                    if (tid < rn) {
+                        const int r = r0 + tid;
+                        m = 0;
+                        while (mrA[m + 1] <= r) ++m;
+                    }
+                    rowmol[tid] = m;
+                } else if (tid < 2 * TS) {
+                    const int t = tid - TS;
+                    int m = -1;
+                    if (t < cn) {
+                        const int r = c0 + t;
+                        m = 0;
+                        while (mrB[m + 1] <= r) ++m;
+                    }
+                    colmol[t] = m;
+                }
+                for (int t = tid; t < TS * MB; t += NT) part[t] = 0.0;
+
+                // ---- Gram sub-tile on MFMA ----
+                v4d c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+                const int lrow = tid >> 2, lk = (tid & 3) * 4;
+                const double *pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + lk;
+                const double *pb = Xp2 + (size_t)(c0 + lrow) * rep_pad + lk;
+                const bool va = lrow < rn, vb = lrow < cn;
+                double2 ra_lo = {0, 0}, ra_hi = {0, 0}, rb_lo = {0, 0}, rb_hi = {0, 0};
+                if (va) { ra_lo = *(const double2 *)pa; ra_hi = *(const double2 *)(pa + 2); }
+                if (vb) { rb_lo = *(const double2 *)pb; rb_hi = *(const double2 *)(pb + 2); }
+                for (int k0 = 0; k0 < rep_pad; k0 += KC) {
+                    __syncthreads();
+                    double *as = As + lrow * LDA + lk;
+                    double *bs = Bs + lrow * LDA + lk;
+                    as[0] = ra_lo.x; as[1] = ra_lo.y; as[2] = ra_hi.x; as[3] = ra_hi.y;
+                    bs[0] = rb_lo.x; bs[1] = rb_lo.y; bs[2] = rb_hi.x; bs[3] = rb_hi.y;
+                    __syncthreads();
+                    if (k0 + KC < rep_pad) {  // prefetch the next chunk
+                        if (va) {
+                            ra_lo = *(const double2 *)(pa + k0 + KC);
+                            ra_hi = *(const double2 *)(pa + k0 + KC + 2);
+                        }
+                        if (vb) {
+                            rb_lo = *(const double2 *)(pb + k0 + KC);
+                            rb_hi = *(const double2 *)(pb + k0 + KC + 2);
+                        }
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < KC / 4; ++ks) {
+                        const int kk = ks * 4 + (lane >> 4);
+                        const double a_0 = As[(wr * 32 + (lane & 15)) * LDA + kk];
+                        const double a_1 = As[(wr * 32 + 16 + (lane & 15)) * LDA + kk];
+                        const double b_0 = Bs[(wc * 32 + (lane & 15)) * LDA + kk];
+                        const double b_1 = Bs[(wc * 32 + 16 + (lane & 15)) * LDA + kk];
+                        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_0, c00, 0, 0, 0);
+                        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_1, c01, 0, 0, 0);
+                        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_0, c10, 0, 0, 0);
+                        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_1, c11, 0, 0, 0);
+                    }
+                }
+                // ---- epilogue: exp into LDS ----
+                const bool dbl_self = diag && sc > sr;  // mirror sub-tile not visited
+#pragma unroll
+                for (int tm = 0; tm < 2; ++tm) {
+#pragma unroll
+                    for (int tn = 0; tn < 2; ++tn) {
+                        const v4d cv = tm == 0 ? (tn == 0 ? c00 : c01) : (tn == 0 ? c10 : c11);
+                        const int col = wc * 32 + tn * 16 + (lane & 15);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int row = wr * 32 + tm * 16 + 4 * (lane >> 4) + v;
+                            double e = 0.0;
+                            if (row < rn && col < cn) {
+                                const double d2 = n2a[r0 + row] + n2b[c0 + col] - 2.0 * cv[v];
+                                e = exp(d2 * cexp);
+                                if (dbl_self && rowmol[row] == colmol[col]) e *= 2.0;
+                            }
+                            Es[row * EP + col] = e;
+                        }
+                    }
+                }
+                __syncthreads();
+                // stage 1: row-wise run sums over column molecules
+                if (tid < TS && tid < rn) {
+                    const double *er = Es + tid * EP;
+                    double *pr = part + tid * MB;
+                    int cur = colmol[0];
+                    double s = 0.0;
+                    for (int j = 0; j < cn; ++j) {
+                        const int m = colmol[j];
+                        if (m != cur) {
+                            pr[cur] = s;
+                            s = 0.0;
+                            cur = m;
+                        }
+                        s += er[j];
+                    }
+                    pr[cur] = s;
+                }
+                __syncthreads();
+                // stage 2: column-molecule owner adds row runs into the accumulator
+                if (tid < MB) {
+                    for (int i = 0; i < rn; ++i) acc[rowmol[i] * MB + tid] += part[i * MB + tid];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- write the block ----
+    if (MODE == SYMM_RFP) {
+        for (int t = tid; t < MB * MB; t += NT) {
+            const int b = t / MB, a = t % MB;
+            if (a < na && b < nb && (!diag || a <= b))
+                out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = acc[a * MB + b];
+        }
+    } else {
+        for (int t = tid; t < MB * MB; t += NT) {
+            const int a = t / MB, b = t % MB;
+            if (a < na && b < nb) {
+                if (MODE == RECT) {
+                    out[(size_t)(a0 + a) * nm2 + b0 + b] = acc[a * MB + b];
+                } else if (!diag || a <= b) {
+                    out[(size_t)(a0 + a) * nm1 + b0 + b] = acc[a * MB + b];
+                    out[(size_t)(b0 + b) * nm1 + a0 + a] = acc[a * MB + b];
+                }
+            }
+        }
+    }
+}
+
+// Device copy of a plan plus the packed rows.
+struct DevicePacked {
+    double *Xp = nullptr, *n2 = nullptr;
+    int *mol_row = nullptr;
+    int rows = 0, rep_pad = 0;
+};
+
+int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &hQ,
+                  std::vector<int> &hN, hipStream_t stream, int slot) {
+    hQ.resize((size_t)nm * ma);
+    hN.resize(nm);
+    int *stage = static_cast<int *>(host_staging(slot, sizeof(int) * ((size_t)nm * ma + nm)));
+    KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
+    if (nm * (size_t)ma)
+        KF_HIP_TRY(hipMemcpyAsync(stage, Q, sizeof(int) * (size_t)nm * ma, hipMemcpyDeviceToHost,
+                                  stream));
+    KF_HIP_TRY(hipMemcpyAsync(stage + (size_t)nm * ma, N, sizeof(int) * nm, hipMemcpyDeviceToHost,
+                              stream));
+    KF_HIP_TRY(hipStreamSynchronize(stream));
+    std::copy(stage, stage + (size_t)nm * ma, hQ.begin());
+    std::copy(stage + (size_t)nm * ma, stage + (size_t)nm * ma + nm, hN.begin());
+    return KF_OK;
+}
+
+int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, int ws_plan,
+                    hipStream_t stream, DevicePacked &d) {
+    d.rows = p.rows;
+    d.rep_pad = (rep + KC - 1) / KC * KC;
+    const size_t mr_n = p.mol_row.size(), src_n = p.src.size();
+    char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * (mr_n + src_n), stream));
+    const size_t pack_bytes = sizeof(double) * ((size_t)p.rows * d.rep_pad + p.rows + 64);
+    char *pack = static_cast<char *>(workspace(ws_pack, pack_bytes, stream));
+    KF_REQUIRE(plan && pack, KF_ERR_NOMEM, "device workspace allocation failed (%zu B)",
+               pack_bytes);
+    d.mol_row = reinterpret_cast<int *>(plan);
+    int *src = d.mol_row + mr_n;
+    int *stage = static_cast<int *>(host_staging(ws_plan, sizeof(int) * (mr_n + src_n)));
+    KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
+    std::copy(p.mol_row.begin(), p.mol_row.end(), stage);
+    std::copy(p.src.begin(), p.src.end(), stage + mr_n);
+    KF_HIP_TRY(hipMemcpyAsync(plan, stage, sizeof(int) * (mr_n + src_n), hipMemcpyHostToDevice,
+                              stream));
+    d.Xp = reinterpret_cast<double *>(pack);
+    d.n2 = d.Xp + (size_t)p.rows * d.rep_pad;
+    if (p.rows > 0) {
+        hipLaunchKernelGGL(pack_rows_f64, dim3((p.rows + 3) / 4), dim3(256), 0, stream, X, src,
+                           p.rows, rep, d.rep_pad, d.Xp, d.n2);
+        KF_HIP_TRY(hipGetLastError());
+    }
+    return KF_OK;
+}
+
+int check_common(const double *X, const int *Q, const int *N, int nm, int ma, int rep,
+                 double sigma) {
+    KF_REQUIRE(nm >= 0 && ma >= 0 && rep >= 1, KF_ERR_ARG, "bad dimensions");
+    KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
+    KF_REQUIRE(nm == 0 || (X && Q && N), KF_ERR_ARG, "null pointer");
+    return KF_OK;
+}
+
+template <int MODE>
+int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm, int ma, int rep,
+              double sigma, void *out, hipStream_t stream) {
+    int rc = check_common(X, Q, N, nm, ma, rep, sigma);
+    if (rc) return rc;
+    KF_REQUIRE(precision == KF_PREC_F64, KF_ERR_ARG, "precision %d not supported here", precision);
+    if (nm == 0) return KF_OK;
+    KF_REQUIRE(out, KF_ERR_ARG, "null output");
+    std::vector<int> hQ, hN;
+    if ((rc = read_metadata(Q, N, nm, ma, hQ, hN, stream, WS_AUX0))) return rc;
+    LocalPlan p;
+    build_plan(hQ.data(), hN.data(), nm, ma, distinct_labels(hQ.data(), hN.data(), nm, ma), p);
+    DevicePacked d;
+    if ((rc = upload_and_pack(p, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
+    const int nblk = (nm + MB - 1) / MB;
+    const long ntile = (long)nblk * (nblk + 1) / 2;
+    const double cexp = -1.0 / (2.0 * sigma * sigma);
+    if (p.nlabels == 0) {  // no atoms at all: zero output
+        const size_t n = MODE == SYMM_RFP ? (size_t)nm * (nm + 1) / 2 : (size_t)nm * nm;
+        KF_HIP_TRY(hipMemsetAsync(out, 0, n * sizeof(double), stream));
+        return KF_OK;
+    }
+    ScopedTimer timer(TM_LOCAL_SYMM, stream);
+    hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3((unsigned)ntile), dim3(NT), 0, stream,
+                       d.Xp, d.n2, d.mol_row, d.Xp, d.n2, d.mol_row, p.nlabels, nm, nm, nblk,
+                       nblk, d.rep_pad, cexp, static_cast<double *>(out));
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
+}
+
+}  // namespace
+}  // namespace kf
+
+extern "C" int kf_local_kernel_symm_rfp(int precision, const double *X, const int *Q,
+                                        const int *N, int nm, int max_atoms, int rep,
+                                        double sigma, void *arf, void *stream) {
+    return kf::symm_impl<kf::SYMM_RFP>(precision, X, Q, N, nm, max_atoms, rep, sigma, arf,
+                                       kf::as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_symm(int precision, const double *X, const int *Q, const int *N,
+                                    int nm, int max_atoms, int rep, double sigma, void *K,
+                                    void *stream) {
+    return kf::symm_impl<kf::SYMM_DENSE>(precision, X, Q, N, nm, max_atoms, rep, sigma, K,
+                                         kf::as_stream(stream));
+}
+
+extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2, const int *Q1,
+                               const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
+                               int max_atoms1, int max_atoms2, int rep, double sigma, void *K,
+                               void *stream_) {
+    using namespace kf;
+    hipStream_t stream = as_stream(stream_);
+    int rc = check_common(X1, Q1, N1, nm1, max_atoms1, rep, sigma);
+    if (rc) return rc;
+    if ((rc = check_common(X2, Q2, N2, nm2, max_atoms2, rep, sigma))) return rc;
+    KF_REQUIRE(precision == KF_PREC_F64, KF_ERR_ARG, "precision %d not supported here", precision);
+    if (nm1 == 0 || nm2 == 0) return KF_OK;
+    KF_REQUIRE(K, KF_ERR_ARG, "null output");
+    std::vector<int> q1, n1, q2, n2;
+    if ((rc = read_metadata(Q1, N1, nm1, max_atoms1, q1, n1, stream, WS_AUX0))) return rc;
+    if ((rc = read_metadata(Q2, N2, nm2, max_atoms2, q2, n2, stream, WS_AUX1))) return rc;
+    std::vector<int> l1 = distinct_labels(q1.data(), n1.data(), nm1, max_atoms1);
+    std::vector<int> l2 = distinct_labels(q2.data(), n2.data(), nm2, max_atoms2);
+    std::vector<int> labels;
+    std::set_intersection(l1.begin(), l1.end(), l2.begin(), l2.end(), std::back_inserter(labels));
+    if (labels.empty()) {
+        KF_HIP_TRY(hipMemsetAsync(K, 0, sizeof(double) * (size_t)nm1 * nm2, stream));
+        return KF_OK;
+    }
+    LocalPlan p1, p2;
+    build_plan(q1.data(), n1.data(), nm1, max_atoms1, labels, p1);
+    build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
+    DevicePacked d1, d2;
+    if ((rc = upload_and_pack(p1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
+    if ((rc = upload_and_pack(p2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
+    const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
+    const double cexp = -1.0 / (2.0 * sigma * sigma);
+    ScopedTimer timer(TM_LOCAL_RECT, stream);
+    hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3((unsigned)((long)nblk1 * nblk2)),
+                       dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d2.Xp, d2.n2, d2.mol_row,
+                       (int)labels.size(), nm1, nm2, nblk1, nblk2, d1.rep_pad, cexp,
+                       static_cast<double *>(K));
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
+}

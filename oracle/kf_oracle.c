@@ -176,19 +176,19 @@ static int two_body_terms(const fchl_params *P, double r, double *val, double *d
     const double inv_rc = P->rcut > 0 ? 1.0 / P->rcut : 0.0;
     const double df = -0.5 * M_PI * inv_rc * sin(M_PI * r * inv_rc);
     const double rp = pow(r, -P->d2);
-    const double pref = f * rp / (s * KFO_SQRT_2PI);
+    const double pref0 = rp / (s * KFO_SQRT_2PI);
     /* d(s^2)/dr, ds/dr, dmu/dr */
     const double ds2 = -2.0 * P->eta2 / (r * (r2 + P->eta2));
     const double ds = ds2 / (2.0 * s);
     const double dmu = 1.0 / r - 0.5 * ds2;
     for (int k = 0; k < P->nRs2; ++k) {
         const double L = log(P->Rs2[k]) - mu;
-        const double g = exp(-0.5 * L * L / s2) / P->Rs2[k];
-        val[k] = pref * g;
+        const double v0 = pref0 * exp(-0.5 * L * L / s2) / P->Rs2[k];
+        val[k] = f * v0;
         if (dval) {
-            const double dlog_g = -ds / s + (L / s2) * dmu + (L * L / (s2 * s)) * ds;
-            const double dlog_pre = df / f - P->d2 / r;
-            dval[k] = val[k] * (dlog_g + dlog_pre);
+            /* d/dr [f v0] = v0 (f dlog(v0)/dr + f'), no division by f */
+            const double dlog0 = -P->d2 / r - ds / s + (L / s2) * dmu + (L * L / (s2 * s)) * ds;
+            dval[k] = v0 * (f * dlog0 + df);
         }
     }
     return 1;

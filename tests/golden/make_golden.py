@@ -1,0 +1,74 @@
+"""Write the golden-vector fixtures of tests/golden/ from the reference's own code.
+
+Run in a container that has the reference tree (it builds oracle/_ref from the
+reference sources src/fchl19_repr.cpp and src/math.cpp with oracle/Makefile):
+
+    python tests/golden/make_golden.py
+
+Fixtures (inputs and the reference's outputs only):
+  fchl19_ref.npz  config-0 molecules (seed 0): coords, Z, generate_fchl_acsf rep for
+                  10 molecules; generate_fchl_acsf_and_gradients (rep, grad) for the 3
+                  smallest; plus one molecule under non-default hyper-parameters.
+  rfp_ref.npz     a 57x57 SPD system in RFP with the reference cho_solve_rfp result.
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from kernelforge_amd import synthetic  # noqa: E402
+
+NONDEFAULT = dict(nRs2=12, nRs3=9, nFourier=2, eta2=0.41, eta3=2.1, rcut=5.5, acut=4.5,
+                  two_body_decay=1.5, three_body_decay=0.7, three_body_weight=9.0,
+                  elements=[1, 6, 7, 8, 16])
+
+
+def main():
+    oracle.build(ref=True)
+    if oracle.ref() is None:
+        raise SystemExit("reference build oracle/_ref is not available")
+    coords, zs, _ = synthetic.random_molecules(40, 0)
+    sel = list(range(10))
+    out = {}
+    for t, m in enumerate(sel):
+        out[f"coords_{t}"] = coords[m]
+        out[f"z_{t}"] = zs[m]
+        out[f"rep_{t}"] = oracle.ref_fchl19(coords[m], zs[m])
+    small = sorted(range(40), key=lambda m: len(zs[m]))[:3]
+    for t, m in enumerate(small):
+        rep, grad = oracle.ref_fchl19(coords[m], zs[m], grad=True)
+        out[f"gcoords_{t}"] = coords[m]
+        out[f"gz_{t}"] = zs[m]
+        out[f"grep_{t}"] = rep
+        out[f"ggrad_{t}"] = grad
+    m = small[0]
+    out["nd_coords"] = coords[m]
+    out["nd_z"] = zs[m]
+    out["nd_rep"] = oracle.ref_fchl19(coords[m], zs[m], **NONDEFAULT)
+    rep, grad = oracle.ref_fchl19(coords[m], zs[m], grad=True, **NONDEFAULT)
+    out["nd_grep"] = rep
+    out["nd_ggrad"] = grad
+    np.savez_compressed(os.path.join(HERE, "fchl19_ref.npz"), **out)
+
+    rng = np.random.default_rng(7)
+    n = 57
+    A = rng.standard_normal((n, n))
+    K = A @ A.T / n + 0.5 * np.eye(n)
+    arf = np.zeros(n * (n + 1) // 2)
+    for i in range(n):
+        for j in range(i, n):
+            arf[oracle.rfp_index(n, i, j)] = K[i, j]
+    y = rng.standard_normal(n)
+    alpha = oracle.ref_cho_solve_rfp(arf, y, 1e-3)
+    np.savez_compressed(os.path.join(HERE, "rfp_ref.npz"), arf=arf, y=y, l2=1e-3, alpha=alpha)
+    print("wrote", os.listdir(HERE))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+"""GPU parity: RFP Cholesky (kf_rfp_potrf / kf_rfp_potrs) and RFP conversions vs LAPACK
+(scipy), numpy.linalg.solve and the reference math.cpp golden."""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def km():
+    from kernelforge_amd import kernelmath
+
+    return kernelmath
+
+
+def spd(n, seed, shift=1.0):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n))
+    return A @ A.T / n + shift * np.eye(n)
+
+
+def to_rfp(K):
+    trttf = sl.get_lapack_funcs("trttf", (K,))
+    arf, info = trttf(np.asfortranarray(K), transr="N", uplo="U")
+    assert info == 0
+    return arf
+
+
+def test_golden(km):
+    g = np.load(os.path.join(ROOT, "tests", "golden", "rfp_ref.npz"))
+    a = km.cho_solve_rfp(g["arf"].copy(), g["y"], l2=float(g["l2"]))
+    assert rel_err(a, g["alpha"]) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 128, 129, 200, 511, 1001])
+def test_cho_solve_rfp_and_factor(km, n):
+    K = spd(n, n)
+    y = np.random.default_rng(1).standard_normal(n)
+    arf = to_rfp(K)
+    work = arf.copy()
+    a = km.cho_solve_rfp(work, y, l2=1e-3)
+    ref = np.linalg.solve(K + 1e-3 * np.eye(n), y)
+    assert rel_err(a, ref) < 1e-10
+    # factor overwrites K_rfp exactly where LAPACK dpftrf leaves it
+    pftrf = sl.get_lapack_funcs("pftrf", (K,))
+    arf_l = arf.copy()
+    for j in range(n):
+        arf_l[oracle.rfp_index(n, j, j)] += 1e-3
+    fac, info = pftrf(n, arf_l, transr="N", uplo="U")
+    assert info == 0
+    assert rel_err(work, fac) < 1e-11
+
+
+def test_not_spd_info_matches_lapack(km):
+    n = 70
+    K = spd(n, 3)
+    K[40, 40] = -5.0
+    arf = to_rfp(K)
+    pftrf = sl.get_lapack_funcs("pftrf", (K,))
+    _, info = pftrf(n, arf.copy(), transr="N", uplo="U")
+    assert info > 0
+    with pytest.raises(RuntimeError, match=f"DPFTRF failed, info={info}"):
+        km.cho_solve_rfp(arf.copy(), np.ones(n), 0.0)
+
+
+def test_solve_cholesky(km):
+    n = 150
+    K = spd(n, 9)
+    y = np.arange(n, dtype=np.float64)
+    K0 = K.copy()
+    a = km.solve_cholesky(K, y, regularize=0.1)
+    assert rel_err(a, np.linalg.solve(K0 + 0.1 * np.eye(n), y)) < 1e-10
+    np.testing.assert_array_equal(K, K0)
+    with pytest.raises(RuntimeError):
+        km.solve_cholesky(np.ones((3, 4)), np.ones(3))
+
+
+@pytest.mark.parametrize("n", [1, 6, 7, 64, 101])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("transr", ["N", "T"])
+def test_rfp_conversions(km, n, uplo, transr):
+    K = spd(n, n + 5)
+    arf = km.full_to_rfp(K, uplo=uplo, transr=transr)
+    # reference semantics: dtrttf on the C-order buffer with UPLO swapped (math_bindings.cpp:89-120)
+    trttf = sl.get_lapack_funcs("trttf", (K,))
+    ref, info = trttf(np.asfortranarray(K.T), transr=transr, uplo="L" if uplo == "U" else "U")
+    np.testing.assert_array_equal(arf, ref)
+    back = km.rfp_to_full(arf, n, uplo=uplo, transr=transr)
+    np.testing.assert_array_equal(back, K)
+
+
+def test_kernel_layout_is_full_to_rfp_lower(km):
+    K = spd(9, 2)
+    np.testing.assert_array_equal(km.full_to_rfp(K, uplo="L"), to_rfp(K))
+
+
+def test_fp32_potrf_potrs_c_abi():
+    import torch
+
+    from kernelforge_amd import capi
+
+    n = 300
+    K = spd(n, 4, shift=2.0)
+    arf = to_rfp(K)
+    y = np.random.default_rng(2).standard_normal(n)
+    dev = torch.device("cuda:0")
+    a32 = torch.from_numpy(arf.astype(np.float32)).to(dev)
+    b = torch.from_numpy(y.copy()).to(dev)
+    capi.rfp_potrf(capi.KF_PREC_F32, a32.data_ptr(), n, 0.0)
+    capi.rfp_potrs(capi.KF_PREC_F32, a32.data_ptr(), n, b.data_ptr(), 1)
+    torch.cuda.synchronize()
+    assert rel_err(b.cpu().numpy(), np.linalg.solve(K, y)) < 1e-4

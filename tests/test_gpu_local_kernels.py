@@ -1,0 +1,85 @@
+"""GPU parity: scalar local Gaussian kernels (kf_local_kernel_symm_rfp / _symm / kf_local_kernel)
+vs the oracle.  Bar: FP64 kernel entries within 1e-10 relative (to max |K|)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lk():
+    from kernelforge_amd import local_kernels
+
+    return local_kernels
+
+
+def random_set(nm, ma, rep, nlab, seed, scale=0.4, nmin=1):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((nm, ma, rep)) * scale
+    q = rng.integers(0, nlab, (nm, ma)).astype(np.int32)
+    n = rng.integers(nmin, ma + 1, nm).astype(np.int32)
+    return x, q, n
+
+
+@pytest.mark.parametrize("nm,ma,rep,nlab", [(1, 3, 7, 2), (13, 5, 9, 3), (33, 4, 16, 1),
+                                            (70, 6, 33, 4), (130, 9, 720, 5)])
+def test_symm_and_rfp_random(lk, nm, ma, rep, nlab):
+    x, q, n = random_set(nm, ma, rep, nlab, nm)
+    Ko = oracle.kernel_symm(x, q, n, 1.3)
+    assert rel_err(lk.kernel_gaussian_symm(x, q, n, 1.3), Ko) < 1e-12
+    assert rel_err(lk.kernel_gaussian_symm_rfp(x, q, n, 1.3), oracle.kernel_symm_rfp(x, q, n, 1.3)) < 1e-12
+
+
+@pytest.mark.parametrize("nm1,nm2", [(1, 1), (5, 40), (33, 65), (100, 7)])
+def test_rect_random(lk, nm1, nm2):
+    x1, q1, n1 = random_set(nm1, 5, 24, 3, nm1)
+    x2, q2, n2 = random_set(nm2, 7, 24, 3, nm2 + 100)
+    K = lk.kernel_gaussian(x1, x2, q1, q2, n1, n2, 0.9)
+    assert K.shape == (nm1, nm2)
+    assert rel_err(K, oracle.kernel_rect(x1, x2, q1, q2, n1, n2, 0.9)) < 1e-12
+
+
+def test_config0_descriptors(lk, config0):
+    _, _, _, C, Q, N = config0
+    X = oracle.fchl19_batch(C, Q, N)
+    arf = lk.kernel_gaussian_symm_rfp(X, Q, N, 2.0)
+    arf_o = oracle.kernel_symm_rfp(X, Q, N, 2.0)
+    assert rel_err(arf, arf_o) < 1e-10
+    Kr = lk.kernel_gaussian(X[800:], X[:800], Q[800:], Q[:800], N[800:], N[:800], 2.0)
+    assert rel_err(Kr, oracle.kernel_rect(X[800:], X[:800], Q[800:], Q[:800], N[800:], N[:800], 2.0)) < 1e-10
+
+
+def test_no_shared_labels_and_clamping(lk):
+    x, q, n = random_set(9, 4, 8, 2, 3)
+    q2 = q + 10
+    K = lk.kernel_gaussian(x, x, q, q2, n, n, 1.0)
+    assert np.all(K == 0.0)
+    n_bad = n.copy()
+    n_bad[0] = 99
+    n_bad[1] = -3
+    assert rel_err(lk.kernel_gaussian_symm(x, q, n_bad, 1.0), oracle.kernel_symm(x, q, n_bad, 1.0)) < 1e-12
+
+
+def test_errors(lk):
+    x, q, n = random_set(4, 3, 5, 2, 1)
+    with pytest.raises(ValueError):
+        lk.kernel_gaussian_symm_rfp(x, q, n, 0.0)
+    with pytest.raises(ValueError):
+        lk.kernel_gaussian_symm_rfp(x, q, n, float("nan"))
+    with pytest.raises(ValueError):
+        lk.kernel_gaussian_symm_rfp(x, q[:, :2], n, 1.0)
+    with pytest.raises(ValueError):
+        lk.kernel_gaussian(x, x[..., :4], q, q, n, n, 1.0)
+
+
+def test_keywords(lk):
+    x, q, n = random_set(6, 3, 5, 2, 8)
+    a = lk.kernel_gaussian_symm_rfp(X=x, Q=q, N=n, sigma=1.1)
+    b = lk.kernel_gaussian_symm_rfp(x, q, n, 1.1)
+    np.testing.assert_array_equal(a, b)
+    c = lk.kernel_gaussian(X1=x, X2=x, Q1=q, Q2=q, N1=n, N2=n, sigma=1.1)
+    np.testing.assert_array_equal(c, lk.kernel_gaussian(x, x, q, q, n, n, 1.1))
