@@ -109,12 +109,9 @@ def main(argv=None):
               "(the path would shard by molecule-pair tiles, see DESIGN.md)", file=sys.stderr)
         return 2
 
-    import torch
-
     from kernelforge_amd import capi, synthetic
 
     L = capi.lib()
-    dev = torch.device("cuda:0")
     name = ctypes_name(L)
     log(f"device {name}")
 
@@ -128,37 +125,35 @@ def main(argv=None):
     ma = Q.shape[1]
     rs = capi.rep_size()
 
-    Ctr = torch.from_numpy(C[:ntr]).to(dev)
-    Qtr = torch.from_numpy(Q[:ntr]).to(dev)
-    Ntr = torch.from_numpy(N[:ntr]).to(dev)
-    ytr = torch.from_numpy(y).to(dev)
-    Xtr = torch.empty((ntr, ma, rs), dtype=torch.float64, device=dev)
-    arf = torch.empty(ntr * (ntr + 1) // 2, dtype=torch.float64, device=dev)
-    alpha = torch.empty(ntr, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream().cuda_stream
+    DA = capi.DeviceArray
+    Ctr, Qtr, Ntr = DA.from_numpy(C[:ntr]), DA.from_numpy(Q[:ntr]), DA.from_numpy(N[:ntr])
+    ytr = DA.from_numpy(y)
+    Xtr = DA((ntr, ma, rs))
+    arf = DA(ntr * (ntr + 1) // 2)
+    alpha = DA(ntr)
+    stream = 0  # the null stream; every call below is ordered on it
 
     def step():
-        capi.fchl19(Ctr.data_ptr(), Qtr.data_ptr(), Ntr.data_ptr(), ntr, ma, Xtr.data_ptr(), 0,
-                    stream)
-        capi.local_kernel_symm_rfp(capi.KF_PREC_F64, Xtr.data_ptr(), Qtr.data_ptr(),
-                                   Ntr.data_ptr(), ntr, ma, rs, SIGMA, arf.data_ptr(), stream)
-        capi.rfp_potrf(capi.KF_PREC_F64, arf.data_ptr(), ntr, LAMBDA, stream)
-        alpha.copy_(ytr)
-        capi.rfp_potrs(capi.KF_PREC_F64, arf.data_ptr(), ntr, alpha.data_ptr(), 1, stream)
+        capi.fchl19(Ctr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, Xtr.ptr, 0, stream)
+        capi.local_kernel_symm_rfp(capi.KF_PREC_F64, Xtr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, rs,
+                                   SIGMA, arf.ptr, stream)
+        capi.rfp_potrf(capi.KF_PREC_F64, arf.ptr, ntr, LAMBDA, stream)
+        alpha.copy_from(ytr, stream)
+        capi.rfp_potrs(capi.KF_PREC_F64, arf.ptr, ntr, alpha.ptr, 1, stream)
 
     for w in range(args.warmup):
         step()
-        torch.cuda.synchronize()
+        capi.synchronize(stream)
         log(f"warmup {w + 1}/{args.warmup} done")
     L.kf_timer_reset()
     L.kf_timer_enable(1)
-    torch.cuda.synchronize()
+    capi.synchronize(stream)
     t0 = time.perf_counter()
     for s in range(args.steps):
         step()
-        torch.cuda.synchronize()
+        capi.synchronize(stream)
         log(f"step {s + 1}/{args.steps} done")
-    torch.cuda.synchronize()
+    capi.synchronize(stream)
     elapsed = time.perf_counter() - t0
     L.kf_timer_enable(0)
     sec_per_fit = elapsed / args.steps
@@ -173,18 +168,15 @@ def main(argv=None):
     achieved = kernel_flop / (avg_ms["kernel"] * 1e-3) / 1e12
 
     # test MAE (after the timed region; not part of the step)
-    Xte = torch.empty((nte, ma, rs), dtype=torch.float64, device=dev)
-    Cte = torch.from_numpy(C[ntr:]).to(dev)
-    Qte = torch.from_numpy(Q[ntr:]).to(dev)
-    Nte = torch.from_numpy(N[ntr:]).to(dev)
-    capi.fchl19(Cte.data_ptr(), Qte.data_ptr(), Nte.data_ptr(), nte, ma, Xte.data_ptr(), 0, stream)
-    Kte = torch.empty((nte, ntr), dtype=torch.float64, device=dev)
-    capi.local_kernel(capi.KF_PREC_F64, Xte.data_ptr(), Xtr.data_ptr(), Qte.data_ptr(),
-                      Qtr.data_ptr(), Nte.data_ptr(), Ntr.data_ptr(), nte, ntr, ma, ma, rs, SIGMA,
-                      Kte.data_ptr(), stream)
-    torch.cuda.synchronize()
+    Xte = DA((nte, ma, rs))
+    Cte, Qte, Nte = DA.from_numpy(C[ntr:]), DA.from_numpy(Q[ntr:]), DA.from_numpy(N[ntr:])
+    capi.fchl19(Cte.ptr, Qte.ptr, Nte.ptr, nte, ma, Xte.ptr, 0, stream)
+    Kte = DA((nte, ntr))
+    capi.local_kernel(capi.KF_PREC_F64, Xte.ptr, Xtr.ptr, Qte.ptr, Qtr.ptr, Nte.ptr, Ntr.ptr, nte,
+                      ntr, ma, ma, rs, SIGMA, Kte.ptr, stream)
+    capi.synchronize(stream)
     cnt_te = np.array([[np.sum(z == e) for e in ELEMENTS] for z in zs[ntr:]], dtype=np.float64)
-    pred = Kte.cpu().numpy() @ alpha.cpu().numpy() + cnt_te @ coef
+    pred = Kte.numpy() @ alpha.numpy() + cnt_te @ coef
     mae = float(np.mean(np.abs(pred - energies[ntr:])))
     log(f"fit {sec_per_fit:.3f} s/step, kernel {avg_ms['kernel']:.1f} ms ({achieved:.1f} TFLOP/s), "
         f"potrf {avg_ms['potrf']:.1f} ms, potrs {avg_ms['potrs']:.1f} ms, test MAE {mae:.4f} Ha")

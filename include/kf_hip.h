@@ -53,6 +53,15 @@ int kf_version(void);
 const char *kf_last_error(void);
 int kf_device_name(char *buf, int len);
 
+/* Device-memory plumbing (no reference counterpart): for FFI callers that hold no
+ * GPU allocator of their own (ctypes, cgo, JNI).  kf_memcpy kind: 0 host->device,
+ * 1 device->host (both complete before returning), 2 device->device (stream-ordered). */
+int kf_malloc(void **ptr, size_t bytes);
+int kf_free(void *ptr);
+int kf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int kf_memset_zero(void *dst, size_t bytes, void *stream);
+int kf_stream_synchronize(void *stream);
+
 /* Kernel timers (instrumentation; no reference counterpart -- the reference prints
  * ad-hoc phase timers, src/local_kernels.cpp:2564-2568).  When enabled, entry
  * points bracket their main kernel launch(es) with HIP events on the launch

@@ -28,7 +28,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kf_runtime.hip", "fchl19.hip", "local_scalar.hip", "rfp_chol.hip",
+HIP_SOURCES = ["kf_runtime.hip", "fchl19.hip", "local_scalar.hip", "local_mixed.hip", "rfp_chol.hip",
                "rfp_convert.hip"]
 PY_MODULES = {"fchl19_repr": "py_fchl19_repr.cpp", "local_kernels": "py_local_kernels.cpp",
               "kernelmath": "py_kernelmath.cpp"}

@@ -1,9 +1,11 @@
 """ctypes binding of the C ABI in ``include/kf_hip.h`` (``libkf_hip.so``).
 
-This is the binding a Python caller holding device memory uses (e.g. torch tensors
-on ``cuda:0``): every array argument is a device pointer passed as an integer
-(``tensor.data_ptr()``), ``stream`` is a ``hipStream_t`` (``torch.cuda.current_stream()
-.cuda_stream``) or 0.  Status codes are turned into the exceptions the reference
+This is the binding a Python caller holding device memory uses: every array argument
+is a device pointer passed as an integer (``DeviceArray.ptr`` below, or any other
+allocation on the same HIP runtime), ``stream`` is a ``hipStream_t`` or 0.  The
+library links the system ROCm runtime; a process must not also load a second,
+differently-versioned HIP runtime (PyTorch wheels bundle their own), so device
+memory here comes from the library's own kf_malloc.  Status codes are turned into the exceptions the reference
 raises (ValueError / RuntimeError / MemoryError).
 """
 
@@ -12,6 +14,8 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+
+import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -28,6 +32,11 @@ SIGNATURES = {
     "kf_version": [],
     "kf_last_error": [],
     "kf_device_name": [ctypes.c_char_p, _c_int],
+    "kf_malloc": [ctypes.POINTER(_vp), ctypes.c_size_t],
+    "kf_free": [_vp],
+    "kf_memcpy": [_vp, _vp, ctypes.c_size_t, _c_int, _vp],
+    "kf_memset_zero": [_vp, ctypes.c_size_t, _vp],
+    "kf_stream_synchronize": [_vp],
     "kf_timer_enable": [_c_int],
     "kf_timer_reset": [],
     "kf_timer_read": [_c_int, ctypes.POINTER(_c_double), ctypes.POINTER(_c_int)],
@@ -81,6 +90,56 @@ def check(rc: int, what: str) -> int:
     if rc > 0:
         raise RuntimeError(f"{what} failed, info={rc}")
     raise RuntimeError(msg)
+
+
+class DeviceArray:
+    """A device allocation made through the C ABI (kf_malloc), with NumPy transfer.
+
+    ``ptr`` is the device address to pass to the kf_* entry points."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = _vp()
+        check(lib().kf_malloc(ctypes.byref(p), self.nbytes), "kf_malloc")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            check(lib().kf_memcpy(d.ptr, a.ctypes.data, d.nbytes, 0, None), "kf_memcpy")
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().kf_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1, None), "kf_memcpy")
+        return out
+
+    def copy_from(self, other, stream=0):
+        assert other.nbytes == self.nbytes
+        check(lib().kf_memcpy(self.ptr, other.ptr, self.nbytes, 2, stream or None), "kf_memcpy")
+
+    def zero(self, stream=0):
+        check(lib().kf_memset_zero(self.ptr, self.nbytes, stream or None), "kf_memset_zero")
+
+    def free(self):
+        if self.ptr and _LIB is not None:
+            _LIB.kf_free(self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(stream=0):
+    check(lib().kf_stream_synchronize(stream or None), "kf_stream_synchronize")
 
 
 def fchl_basis(elements=(1, 6, 7, 8, 16), nRs2=24, nRs3=20, nFourier=1, rcut=8.0, acut=8.0,

@@ -452,44 +452,47 @@ __global__ __launch_bounds__(kGradThreads) void fchl19_grad_kernel(
     __syncthreads();
 
     // ---------------- three-body blocks, one channel at a time ----------------
-    const int nslot = P.nb3 * P.na;  // <= 64 enforced on the host
+    const int nslot = P.nb3 * P.na;
     for (int ch = 0; ch < P.nchan; ++ch) {
         const int lo = ch == 0 ? 0 : chan_cnt[ch - 1], hi = chan_cnt[ch];
-        for (int t = threadIdx.x; t < nslot * ld; t += blockDim.x) blk[t] = 0.0;
-        __syncthreads();
-        for (int item = threadIdx.x; item < nslot * 3; item += blockDim.x) {
-            const int s = item / 3, d = item % 3;
-            const int l = s / P.na, ta = s % P.na;
-            const double rs3 = P.Rs3[l];
-            double val = 0.0;
-            double *row = blk + (size_t)s * ld;
-            for (int q = lo; q < hi; ++q) {
-                const int tr = chan_list[q];
-                const double *rr = recs + (size_t)tr * kRecW;
-                const int j = tri_jk[2 * tr], k = tri_jk[2 * tr + 1];
-                const double dr = rr[30] - rs3;
-                const double rad = exp(-P.eta3 * dr * dr);
-                const double drad = -2.0 * P.eta3 * dr * rad;
-                const double a = ta < 2 ? rr[ta] : 0.0;
-                const double W = rr[20];
-                if (d == 0) val += a * rad * W;
-                if (ta < 2) {
-                    const double *ga = rr + (ta == 0 ? 2 : 11);
-                    const int atom[3] = {i, j, k};
+        for (int s0 = 0; s0 < nslot; s0 += 64) {  // slot chunks of the LDS block
+            const int ns = (nslot - s0) < 64 ? (nslot - s0) : 64;
+            for (int t = threadIdx.x; t < ns * ld; t += blockDim.x) blk[t] = 0.0;
+            __syncthreads();
+            for (int item = threadIdx.x; item < ns * 3; item += blockDim.x) {
+                const int sl = item / 3, d = item % 3, s = s0 + sl;
+                const int l = s / P.na, ta = s % P.na;
+                const double rs3 = P.Rs3[l];
+                double val = 0.0;
+                double *row = blk + (size_t)sl * ld;
+                for (int q = lo; q < hi; ++q) {
+                    const int tr = chan_list[q];
+                    const double *rr = recs + (size_t)tr * kRecW;
+                    const int j = tri_jk[2 * tr], k = tri_jk[2 * tr + 1];
+                    const double dr = rr[30] - rs3;
+                    const double rad = exp(-P.eta3 * dr * dr);
+                    const double drad = -2.0 * P.eta3 * dr * rad;
+                    const double a = ta < 2 ? rr[ta] : 0.0;
+                    const double W = rr[20];
+                    if (d == 0) val += a * rad * W;
+                    if (ta < 2) {
+                        const double *ga = rr + (ta == 0 ? 2 : 11);
+                        const int atom[3] = {i, j, k};
 #pragma unroll
-                    for (int role = 0; role < 3; ++role) {
-                        const int c = 3 * role + d;
-                        row[3 * atom[role] + d] +=
-                            rad * (ga[c] * W + a * rr[21 + c]) + a * W * drad * rr[31 + c];
+                        for (int role = 0; role < 3; ++role) {
+                            const int c = 3 * role + d;
+                            row[3 * atom[role] + d] +=
+                                rad * (ga[c] * W + a * rr[21 + c]) + a * W * drad * rr[31 + c];
+                        }
                     }
                 }
+                if (d == 0) Xi[P.three_off + ch * nslot + s] = val;
             }
-            if (d == 0) Xi[P.three_off + ch * nslot + s] = val;
+            __syncthreads();
+            double *dst = dXi + (size_t)(P.three_off + ch * nslot + s0) * ld;
+            for (int t = threadIdx.x; t < ns * ld; t += blockDim.x) dst[t] = blk[t];
+            __syncthreads();
         }
-        __syncthreads();
-        double *dst = dXi + (size_t)(P.three_off + ch * nslot) * ld;
-        for (int t = threadIdx.x; t < nslot * ld; t += blockDim.x) dst[t] = blk[t];
-        __syncthreads();
     }
 }
 
@@ -570,8 +573,6 @@ extern "C" int kf_fchl19_f64(const double *coords, const int *Q, const int *N, i
         hipLaunchKernelGGL(fchl19_rep_kernel, dim3(nm), dim3(kRepThreads), lds, stream, coords, Q,
                            N, P, X, err);
     } else {
-        KF_REQUIRE(P.nb3 * P.na <= 64, KF_ERR_BASIS,
-                   "gradient path supports nRs3*nabasis <= 64");
         KF_REQUIRE(P.nelem * P.nb2 > 0, KF_ERR_BASIS, "empty two-body basis");
         const int ma = max_atoms, ld = 3 * ma;
         const int ntri_cap = ma * (ma - 1) / 2;

@@ -102,4 +102,15 @@ __host__ __device__ inline size_t rfp_upper(size_t n, size_t i, size_t j) {
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Row of result register v held by `lane` in a 16x16x4 MFMA (column = lane & 15).
+// Both take A[i][k] from lane i + 16k and B[k][j] from lane j + 16k, but they place
+// the 16x16 result differently: the FP64 form interleaves the four registers across
+// row groups (row = 4v + lane/16), the FP32 form keeps them consecutive
+// (row = 4*(lane/16) + v).
+__device__ inline int mfma16_row_f64(int lane, int v) { return 4 * v + (lane >> 4); }
+__device__ inline int mfma16_row_f32(int lane, int v) { return 4 * (lane >> 4) + v; }
+template <typename T> __device__ inline int mfma16_row(int lane, int v);
+template <> __device__ inline int mfma16_row<double>(int lane, int v) { return mfma16_row_f64(lane, v); }
+template <> __device__ inline int mfma16_row<float>(int lane, int v) { return mfma16_row_f32(lane, v); }
+
 }  // namespace kf

@@ -134,6 +134,52 @@ extern "C" int kf_timer_read(int id, double *total_ms, int *count) {
     return KF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device-memory plumbing for FFI callers that hold no GPU allocator of their own.
+// ---------------------------------------------------------------------------
+extern "C" int kf_malloc(void **ptr, size_t bytes) {
+    KF_REQUIRE(ptr, KF_ERR_ARG, "null out-pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return KF_OK;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        kf::set_last_error("hipMalloc of %zu bytes failed", bytes);
+        return KF_ERR_NOMEM;
+    }
+    return KF_OK;
+}
+
+extern "C" int kf_free(void *ptr) {
+    if (ptr) KF_HIP_TRY(hipFree(ptr));
+    return KF_OK;
+}
+
+// kind: 0 host->device, 1 device->host, 2 device->device.  Host<->device copies
+// complete before the call returns; device->device copies are stream-ordered.
+extern "C" int kf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+    if (bytes == 0) return KF_OK;
+    KF_REQUIRE(dst && src && kind >= 0 && kind <= 2, KF_ERR_ARG, "bad kf_memcpy arguments");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                      : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    hipStream_t s = kf::as_stream(stream);
+    KF_HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, s));
+    if (kind != 2) KF_HIP_TRY(hipStreamSynchronize(s));
+    return KF_OK;
+}
+
+extern "C" int kf_memset_zero(void *dst, size_t bytes, void *stream) {
+    if (bytes == 0) return KF_OK;
+    KF_REQUIRE(dst, KF_ERR_ARG, "null pointer");
+    KF_HIP_TRY(hipMemsetAsync(dst, 0, bytes, kf::as_stream(stream)));
+    return KF_OK;
+}
+
+extern "C" int kf_stream_synchronize(void *stream) {
+    KF_HIP_TRY(hipStreamSynchronize(kf::as_stream(stream)));
+    return KF_OK;
+}
+
 extern "C" int kf_version(void) { return 1; }
 
 extern "C" const char *kf_last_error(void) { return kf::g_last_error.c_str(); }

@@ -20,8 +20,7 @@
 //    written exactly once, straight into RFP (UPLO='U') or dense layout.
 #include <cmath>
 
-#include "kf_common.hpp"
-#include "local_plan.hpp"
+#include "local_mixed.hpp"
 
 namespace kf {
 namespace {
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                         const int col = wc * 32 + tn * 16 + (lane & 15);
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const int row = wr * 32 + tm * 16 + 4 * (lane >> 4) + v;
+                            const int row = wr * 32 + tm * 16 + mfma16_row_f64(lane, v);
                             double e = 0.0;
                             if (row < rn && col < cn) {
                                 const double d2 = n2a[r0 + row] + n2b[c0 + col] - 2.0 * cv[v];
@@ -307,13 +306,16 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
               double sigma, void *out, hipStream_t stream) {
     int rc = check_common(X, Q, N, nm, ma, rep, sigma);
     if (rc) return rc;
-    KF_REQUIRE(precision == KF_PREC_F64, KF_ERR_ARG, "precision %d not supported here", precision);
+    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3, KF_ERR_ARG,
+               "precision %d not supported here", precision);
     if (nm == 0) return KF_OK;
     KF_REQUIRE(out, KF_ERR_ARG, "null output");
     std::vector<int> hQ, hN;
     if ((rc = read_metadata(Q, N, nm, ma, hQ, hN, stream, WS_AUX0))) return rc;
     LocalPlan p;
     build_plan(hQ.data(), hN.data(), nm, ma, distinct_labels(hQ.data(), hN.data(), nm, ma), p);
+    if (precision == KF_PREC_BF16X3)
+        return local_mixed_symm(MODE == SYMM_DENSE, X, nm, ma, rep, sigma, p, out, stream);
     DevicePacked d;
     if ((rc = upload_and_pack(p, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
     const int nblk = (nm + MB - 1) / MB;
@@ -358,7 +360,8 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     int rc = check_common(X1, Q1, N1, nm1, max_atoms1, rep, sigma);
     if (rc) return rc;
     if ((rc = check_common(X2, Q2, N2, nm2, max_atoms2, rep, sigma))) return rc;
-    KF_REQUIRE(precision == KF_PREC_F64, KF_ERR_ARG, "precision %d not supported here", precision);
+    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3, KF_ERR_ARG,
+               "precision %d not supported here", precision);
     if (nm1 == 0 || nm2 == 0) return KF_OK;
     KF_REQUIRE(K, KF_ERR_ARG, "null output");
     std::vector<int> q1, n1, q2, n2;
@@ -368,13 +371,17 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     std::vector<int> l2 = distinct_labels(q2.data(), n2.data(), nm2, max_atoms2);
     std::vector<int> labels;
     std::set_intersection(l1.begin(), l1.end(), l2.begin(), l2.end(), std::back_inserter(labels));
+    const size_t esz = precision == KF_PREC_F64 ? sizeof(double) : sizeof(float);
     if (labels.empty()) {
-        KF_HIP_TRY(hipMemsetAsync(K, 0, sizeof(double) * (size_t)nm1 * nm2, stream));
+        KF_HIP_TRY(hipMemsetAsync(K, 0, esz * (size_t)nm1 * nm2, stream));
         return KF_OK;
     }
     LocalPlan p1, p2;
     build_plan(q1.data(), n1.data(), nm1, max_atoms1, labels, p1);
     build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
+    if (precision == KF_PREC_BF16X3)
+        return local_mixed_rect(X1, X2, nm1, nm2, max_atoms1, max_atoms2, rep, sigma, p1, p2,
+                                static_cast<float *>(K), stream);
     DevicePacked d1, d2;
     if ((rc = upload_and_pack(p1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
     if ((rc = upload_and_pack(p2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;

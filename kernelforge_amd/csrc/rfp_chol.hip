@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void trsm_panel(T *a, Part P, int J, const T *
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int v = 0; v < 4; ++v)
-                Pn[(wr * 32 + i * 16 + 4 * (lane >> 4) + v) * TP + wc * 32 + j * 16 + (lane & 15)] = acc[i][j][v];
+                Pn[(wr * 32 + i * 16 + mfma16_row<T>(lane, v)) * TP + wc * 32 + j * 16 + (lane & 15)] = acc[i][j][v];
     __syncthreads();
     store_tile<T, false>(a, t, rn, w, Pn);
 }
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void syrk_update(T *a, Part P, int J, const in
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int v = 0; v < 4; ++v)
-                C[(wr * 32 + i * 16 + 4 * (lane >> 4) + v) * TP + wc * 32 + j * 16 + (lane & 15)] -= acc[i][j][v];
+                C[(wr * 32 + i * 16 + mfma16_row<T>(lane, v)) * TP + wc * 32 + j * 16 + (lane & 15)] -= acc[i][j][v];
     __syncthreads();
     if (diagt) store_tile<T, true>(a, tc, rni, rnk, C); else store_tile<T, false>(a, tc, rni, rnk, C);
 }

@@ -87,21 +87,17 @@ def test_flat_coords_and_errors(fr, config0):
 
 
 def test_c_abi_device_resident(config0):
-    import torch
-
     from kernelforge_amd import capi
 
     _, _, _, C, Q, N = config0
-    nm, ma = Q.shape
-    dev = torch.device("cuda:0")
-    tC = torch.from_numpy(C[:64]).to(dev)
-    tQ = torch.from_numpy(Q[:64]).to(dev)
-    tN = torch.from_numpy(N[:64]).to(dev)
-    X = torch.empty((64, ma, 720), dtype=torch.float64, device=dev)
-    dX = torch.empty((64, ma, 720, 3 * ma), dtype=torch.float64, device=dev)
-    s = torch.cuda.current_stream().cuda_stream
-    capi.fchl19(tC.data_ptr(), tQ.data_ptr(), tN.data_ptr(), 64, ma, X.data_ptr(), dX.data_ptr(), s)
-    torch.cuda.synchronize()
+    ma = Q.shape[1]
+    dC = capi.DeviceArray.from_numpy(C[:64])
+    dQ = capi.DeviceArray.from_numpy(Q[:64])
+    dN = capi.DeviceArray.from_numpy(N[:64])
+    X = capi.DeviceArray((64, ma, 720))
+    dX = capi.DeviceArray((64, ma, 720, 3 * ma))
+    capi.fchl19(dC.ptr, dQ.ptr, dN.ptr, 64, ma, X.ptr, dX.ptr)
+    capi.synchronize()
     Xo, dXo = oracle.fchl19_batch(C[:64], Q[:64], N[:64], grad=True)
-    assert row_rel_err(X.cpu().numpy(), Xo, lead=2) < 1e-12
-    assert row_rel_err(dX.cpu().numpy(), dXo, lead=2) < 1e-12
+    assert row_rel_err(X.numpy(), Xo, lead=2) < 1e-12
+    assert row_rel_err(dX.numpy(), dXo, lead=2) < 1e-12

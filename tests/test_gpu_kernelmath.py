@@ -103,18 +103,15 @@ def test_kernel_layout_is_full_to_rfp_lower(km):
 
 
 def test_fp32_potrf_potrs_c_abi():
-    import torch
-
     from kernelforge_amd import capi
 
     n = 300
     K = spd(n, 4, shift=2.0)
     arf = to_rfp(K)
     y = np.random.default_rng(2).standard_normal(n)
-    dev = torch.device("cuda:0")
-    a32 = torch.from_numpy(arf.astype(np.float32)).to(dev)
-    b = torch.from_numpy(y.copy()).to(dev)
-    capi.rfp_potrf(capi.KF_PREC_F32, a32.data_ptr(), n, 0.0)
-    capi.rfp_potrs(capi.KF_PREC_F32, a32.data_ptr(), n, b.data_ptr(), 1)
-    torch.cuda.synchronize()
-    assert rel_err(b.cpu().numpy(), np.linalg.solve(K, y)) < 1e-4
+    a32 = capi.DeviceArray.from_numpy(arf.astype(np.float32))
+    b = capi.DeviceArray.from_numpy(y)
+    capi.rfp_potrf(capi.KF_PREC_F32, a32.ptr, n, 0.0)
+    capi.rfp_potrs(capi.KF_PREC_F32, a32.ptr, n, b.ptr, 1)
+    capi.synchronize()
+    assert rel_err(b.numpy(), np.linalg.solve(K, y)) < 1e-4

@@ -83,3 +83,28 @@ def test_keywords(lk):
     np.testing.assert_array_equal(a, b)
     c = lk.kernel_gaussian(X1=x, X2=x, Q1=q, Q2=q, N1=n, N2=n, sigma=1.1)
     np.testing.assert_array_equal(c, lk.kernel_gaussian(x, x, q, q, n, n, 1.1))
+
+
+# ---------------------------------------------------------------------------
+# BF16x3 split-precision path (precision='bf16x3'): accuracy bar vs the FP64 oracle.
+# Tolerances: max |K_mixed - K_fp64| <= 2e-5 * max |K| (measured ~3e-6 in emulation).
+# ---------------------------------------------------------------------------
+def test_bf16x3_symm_config0(lk, config0):
+    _, _, _, C, Q, N = config0
+    X = oracle.fchl19_batch(C[:400], Q[:400], N[:400])
+    K64 = oracle.kernel_symm(X, Q[:400], N[:400], 2.0)
+    K = lk.kernel_gaussian_symm(X, Q[:400], N[:400], 2.0, precision="bf16x3")
+    assert K.dtype == np.float32
+    assert rel_err(K, K64) < 2e-5
+    arf = lk.kernel_gaussian_symm_rfp(X, Q[:400], N[:400], 2.0, precision="bf16x3")
+    assert rel_err(arf, oracle.kernel_symm_rfp(X, Q[:400], N[:400], 2.0)) < 2e-5
+
+
+def test_bf16x3_rect_and_random(lk, config0):
+    _, _, _, C, Q, N = config0
+    X = oracle.fchl19_batch(C[:300], Q[:300], N[:300])
+    Kr = lk.kernel_gaussian(X[:70], X[70:], Q[:70], Q[70:], N[:70], N[70:], 2.0, precision="bf16x3")
+    assert rel_err(Kr, oracle.kernel_rect(X[:70], X[70:], Q[:70], Q[70:], N[:70], N[70:], 2.0)) < 2e-5
+    x, q, n = random_set(150, 6, 33, 3, 5)
+    assert rel_err(lk.kernel_gaussian_symm(x, q, n, 1.3, precision="bf16x3"),
+                   oracle.kernel_symm(x, q, n, 1.3)) < 2e-5
