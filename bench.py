@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("KRR solve s @≤1e-6 rel err vs FP64: plain vs mixed+learned PCG; kernel TFLOP/s")
 PEAK_F64_MFMA_TFLOPS = 78.6
+PEAK_BF16_MFMA_TFLOPS = 2500.0
 SIGMA, LAMBDA = 2.0, 1e-8
 ELEMENTS = (1, 6, 7, 8, 16)
 
@@ -103,6 +104,9 @@ def main(argv=None):
     ap.add_argument("--n-test", type=int, default=2000)
     ap.add_argument("--cpu-sample", type=int, default=3000,
                     help="molecules in the CPU-baseline sample (0 skips it)")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
+                    help="fp64: plain path; mixed: BF16x3 kernel + FP32 factor + PCG; both: "
+                         "time both, report the mixed fit with its error vs the FP64 fit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -133,7 +137,12 @@ def main(argv=None):
     alpha = DA(ntr)
     stream = 0  # the null stream; every call below is ordered on it
 
-    def step():
+    arf32 = DA(ntr * (ntr + 1) // 2, np.float32) if args.precision != "fp64" else None
+    lwork = DA(ntr * (ntr + 1) // 2, np.float32) if args.precision != "fp64" else None
+    alpha_m = DA(ntr) if args.precision != "fp64" else None
+    pcg_info = {}
+
+    def step_fp64():
         capi.fchl19(Ctr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, Xtr.ptr, 0, stream)
         capi.local_kernel_symm_rfp(capi.KF_PREC_F64, Xtr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, rs,
                                    SIGMA, arf.ptr, stream)
@@ -141,45 +150,96 @@ def main(argv=None):
         alpha.copy_from(ytr, stream)
         capi.rfp_potrs(capi.KF_PREC_F64, arf.ptr, ntr, alpha.ptr, 1, stream)
 
-    for w in range(args.warmup):
-        step()
+    def step_mixed():
+        capi.fchl19(Ctr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, Xtr.ptr, 0, stream)
+        capi.local_kernel_symm_rfp(capi.KF_PREC_BF16X3, Xtr.ptr, Qtr.ptr, Ntr.ptr, ntr, ma, rs,
+                                   SIGMA, arf32.ptr, stream)
+        it, rel, mu = capi.krr_solve_mixed(arf32.ptr, lwork.ptr, ntr, LAMBDA, ytr.ptr,
+                                           alpha_m.ptr, tol=1e-10, max_iter=300, stream=stream)
+        pcg_info.update(iters=it, rel_residual=rel, lambda_solved=mu)
+
+    def timed(step, label):
+        for w in range(args.warmup):
+            step()
+            capi.synchronize(stream)
+            log(f"{label}: warmup {w + 1}/{args.warmup} done")
+        L.kf_timer_reset()
+        L.kf_timer_enable(1)
         capi.synchronize(stream)
-        log(f"warmup {w + 1}/{args.warmup} done")
-    L.kf_timer_reset()
-    L.kf_timer_enable(1)
-    capi.synchronize(stream)
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        step()
+        t0 = time.perf_counter()
+        for s_ in range(args.steps):
+            step()
+            capi.synchronize(stream)
+            log(f"{label}: step {s_ + 1}/{args.steps} done")
         capi.synchronize(stream)
-        log(f"step {s + 1}/{args.steps} done")
-    capi.synchronize(stream)
-    elapsed = time.perf_counter() - t0
-    L.kf_timer_enable(0)
-    sec_per_fit = elapsed / args.steps
-    timers = {k: capi.timer_read(v) for k, v in
+        el = time.perf_counter() - t0
+        L.kf_timer_enable(0)
+        tm = {k: capi.timer_read(v) for k, v in
               (("fchl19", capi.TM_FCHL19), ("kernel", capi.TM_LOCAL_SYMM),
-               ("potrf", capi.TM_POTRF), ("potrs", capi.TM_POTRS))}
-    avg_ms = {k: (ms / c if c else None) for k, (ms, c) in timers.items()}
+               ("potrf", capi.TM_POTRF), ("potrs", capi.TM_POTRS), ("pcg", capi.TM_MATVEC))}
+        return el / args.steps, {k: (ms / c if c else None) for k, (ms, c) in tm.items()}
+
+    res = {}
+    if args.precision in ("fp64", "both"):
+        res["fp64"] = timed(step_fp64, "fp64")
+    if args.precision in ("mixed", "both"):
+        try:
+            res["mixed"] = timed(step_mixed, "mixed")
+        except RuntimeError as exc:  # reported, the plain path still stands
+            log(f"mixed path failed: {exc}")
+            if not res:
+                raise
+    head = "mixed" if "mixed" in res else "fp64"
+    sec_per_fit, avg_ms = res[head]
 
     rows = label_rows(Q[:ntr], N[:ntr])
     rows_sq = sum(v * v for v in rows.values())
     kernel_flop = rs * rows_sq  # symmetric half of 2 * R_e^2 * rep, summed over labels
-    achieved = kernel_flop / (avg_ms["kernel"] * 1e-3) / 1e12
+    if head == "mixed":  # three BF16 products per contraction term
+        roof_flop, peak, kname = 3 * kernel_flop, PEAK_BF16_MFMA_TFLOPS, "local_bf16x3_kernel<SYMM_RFP>"
+    else:
+        roof_flop, peak, kname = kernel_flop, PEAK_F64_MFMA_TFLOPS, "local_scalar_f64_kernel<SYMM_RFP>"
+    achieved = roof_flop / (avg_ms["kernel"] * 1e-3) / 1e12
 
-    # test MAE (after the timed region; not part of the step)
+    # test predictions and errors (after the timed regions; not part of a step)
     Xte = DA((nte, ma, rs))
     Cte, Qte, Nte = DA.from_numpy(C[ntr:]), DA.from_numpy(Q[ntr:]), DA.from_numpy(N[ntr:])
     capi.fchl19(Cte.ptr, Qte.ptr, Nte.ptr, nte, ma, Xte.ptr, 0, stream)
+    cnt_te = np.array([[np.sum(z == e) for e in ELEMENTS] for z in zs[ntr:]], dtype=np.float64)
     Kte = DA((nte, ntr))
     capi.local_kernel(capi.KF_PREC_F64, Xte.ptr, Xtr.ptr, Qte.ptr, Qtr.ptr, Nte.ptr, Ntr.ptr, nte,
                       ntr, ma, ma, rs, SIGMA, Kte.ptr, stream)
     capi.synchronize(stream)
-    cnt_te = np.array([[np.sum(z == e) for e in ELEMENTS] for z in zs[ntr:]], dtype=np.float64)
-    pred = Kte.numpy() @ alpha.numpy() + cnt_te @ coef
-    mae = float(np.mean(np.abs(pred - energies[ntr:])))
-    log(f"fit {sec_per_fit:.3f} s/step, kernel {avg_ms['kernel']:.1f} ms ({achieved:.1f} TFLOP/s), "
-        f"potrf {avg_ms['potrf']:.1f} ms, potrs {avg_ms['potrs']:.1f} ms, test MAE {mae:.4f} Ha")
+    Kte_h = Kte.numpy()
+    accuracy = {}
+    mae = None
+    if "fp64" in res:
+        a64 = alpha.numpy()
+        pred64 = Kte_h @ a64 + cnt_te @ coef
+        mae = float(np.mean(np.abs(pred64 - energies[ntr:])))
+    if "mixed" in res:
+        Kte32 = DA((nte, ntr), np.float32)
+        capi.local_kernel(capi.KF_PREC_BF16X3, Xte.ptr, Xtr.ptr, Qte.ptr, Qtr.ptr, Nte.ptr,
+                          Ntr.ptr, nte, ntr, ma, ma, rs, SIGMA, Kte32.ptr, stream)
+        capi.synchronize(stream)
+        am = alpha_m.numpy()
+        predm = Kte32.numpy().astype(np.float64) @ am + cnt_te @ coef
+        accuracy.update(pcg_iters=pcg_info.get("iters"),
+                        pcg_rel_residual=pcg_info.get("rel_residual"),
+                        lambda_solved=pcg_info.get("lambda_solved"),
+                        test_mae_mixed_ha=float(np.mean(np.abs(predm - energies[ntr:]))))
+        if "fp64" in res:
+            accuracy.update(
+                energy_rel_err_max=float(np.max(np.abs(predm - pred64) / np.abs(pred64))),
+                energy_abs_err_max_ha=float(np.max(np.abs(predm - pred64))),
+                alpha_rel_err_inf=float(np.max(np.abs(am - a64)) / np.max(np.abs(a64))),
+                kernel_rel_err_test_block=float(np.max(np.abs(Kte32.numpy() - Kte_h)) /
+                                                np.max(np.abs(Kte_h))),
+                speedup_vs_fp64=res["fp64"][0] / res["mixed"][0])
+    log(f"{head} fit {sec_per_fit:.3f} s/step, kernel {avg_ms['kernel']:.1f} ms "
+        f"({achieved:.1f} TFLOP/s), breakdown {avg_ms}")
+    if accuracy:
+        log(f"accuracy {accuracy}")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -198,31 +258,41 @@ def main(argv=None):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3/f32/f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"FCHL19 local-kernel KRR energy fit, plain FP64: {ntr} synthetic "
-                         f"molecules (seed 1), descriptors + symmetric RFP kernel + RFP "
-                         f"Cholesky solve"),
+            "workload": (f"FCHL19 local-kernel KRR energy fit, {ntr} synthetic molecules "
+                         f"(seed 1): descriptors + symmetric RFP kernel + solve; "
+                         + ("mixed: BF16x3 MFMA kernel (FP32 RFP), FP32 RFP Cholesky, FP64-"
+                            "residual PCG to 1e-10" if head == "mixed" else
+                            "plain FP64 kernel and RFP Cholesky")),
             "n_train": ntr, "n_test": nte, "max_atoms": ma, "rep": rs, "sigma": SIGMA,
-            "lambda": LAMBDA, "precision": "fp64",
+            "lambda": LAMBDA, "precision": head,
         },
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 3),
-            "peak": PEAK_F64_MFMA_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_F64_MFMA_TFLOPS, 4),
+            "frac": round(achieved / peak, 4),
             "traffic": None,
-            "kernel": "local_scalar_f64_kernel<SYMM_RFP>",
-            "flop_per_launch": kernel_flop,
+            "kernel": kname,
+            "flop_per_launch": roof_flop,
             "avg_launch_ms": round(avg_ms["kernel"], 3),
         },
         "cpu_baseline": cpu,
         "breakdown_ms": {k: (round(v, 3) if v is not None else None) for k, v in avg_ms.items()},
-        "test_mae_ha": round(mae, 5),
         "device": name,
     }
+    if "fp64" in res:
+        out["plain_fp64"] = {"s_per_fit": round(res["fp64"][0], 4),
+                             "breakdown_ms": {k: (round(v, 3) if v is not None else None)
+                                              for k, v in res["fp64"][1].items()},
+                             "kernel_tflops": round(kernel_flop / (res["fp64"][1]["kernel"] * 1e-3)
+                                                    / 1e12, 3),
+                             "test_mae_ha": round(mae, 5)}
+    if accuracy:
+        out["mixed_accuracy_vs_fp64"] = accuracy
     print(json.dumps(out), flush=True)
     return 0
 

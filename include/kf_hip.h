@@ -135,6 +135,34 @@ int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *stream);
  * double b: the solve accumulates in FP64). */
 int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream);
 
+/* y = (A + lambda I) x for a symmetric RFP matrix A (precision KF_PREC_F64: double*,
+ * KF_PREC_F32: float*), FP64 vectors and accumulation, deterministic.  The operator
+ * of the refinement residual r = y - (K + lambda I) alpha (no reference counterpart:
+ * the reference only factorises, src/math.cpp:134-171). */
+int kf_rfp_symv(int precision, const void *arf, int n, double lambda, const double *x, double *y,
+                void *stream);
+
+/* Preconditioned conjugate gradients on (A + lambda I) alpha = y, A an RFP matrix
+ * (a_precision F64/F32), preconditioner M = L L^T from an RFP Cholesky factor
+ * (m_precision F64/F32) of a shifted A.  alpha holds the initial guess on entry.
+ * Stops at ||r|| / ||y|| <= tol (true FP64 residual); returns KF_ERR_NOCONV when
+ * max_iter is reached first.  No reference counterpart (the reference's GPU model
+ * runs a Jacobi-PCG on dense FP32 K, models/cuda_local_krr.py:214-287). */
+int kf_rfp_pcg(int a_precision, const void *arf, int m_precision, const void *lfac, int n,
+               double lambda, const double *y, double *alpha, double tol, int max_iter, int *iters,
+               double *rel_residual, void *stream);
+
+/* Mixed-precision replacement of cho_solve_rfp (src/math.cpp:134-171) for an FP32
+ * RFP kernel K32 (e.g. from kf_local_kernel_symm_rfp with KF_PREC_BF16X3): FP32 RFP
+ * Cholesky of K32 + mu I into lwork (n(n+1)/2 floats; mu = lambda, raised only if
+ * the FP32 factorisation breaks down) and PCG to ||r||/||y|| <= tol with FP64
+ * residuals of (K32 + lambda I).  When K32 + lambda I is not positive definite to
+ * working precision (PCG breakdown) the system solved is K32 + mu I; *lambda_used
+ * returns the regularisation actually solved.  K32 is not modified. */
+int kf_krr_solve_mixed(const float *K32, float *lwork, int n, double lambda, const double *y,
+                       double *alpha, double tol, int max_iter, int *iters, double *rel_residual,
+                       double *lambda_used, void *stream);
+
 /* Full (column-major, leading dimension lda) -> RFP, LAPACK dtrttf semantics for
  * all TRANSR in {'N','T'} x UPLO in {'U','L'}.  Replaces the dtrttf_ call of
  * kf::math::full_to_rfp (src/math.cpp:173-179); kernelmath.full_to_rfp passes the

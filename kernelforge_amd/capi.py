@@ -50,6 +50,12 @@ SIGNATURES = {
                         _c_int, _c_double, _vp, _vp],
     "kf_rfp_potrf": [_c_int, _vp, _c_int, _c_double, _vp],
     "kf_rfp_potrs": [_c_int, _vp, _c_int, _vp, _c_int, _vp],
+    "kf_rfp_symv": [_c_int, _vp, _c_int, _c_double, _vp, _vp, _vp],
+    "kf_rfp_pcg": [_c_int, _vp, _c_int, _vp, _c_int, _c_double, _vp, _vp, _c_double, _c_int,
+                   ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp],
+    "kf_krr_solve_mixed": [_vp, _vp, _c_int, _c_double, _vp, _vp, _c_double, _c_int,
+                           ctypes.POINTER(_c_int), ctypes.POINTER(_c_double),
+                           ctypes.POINTER(_c_double), _vp],
     "kf_trttf": [_c_char, _c_char, _c_int, _vp, _c_int, _vp, _vp],
     "kf_tfttr": [_c_char, _c_char, _c_int, _vp, _vp, _c_int, _c_int, _vp],
 }
@@ -190,3 +196,16 @@ def rfp_potrf(precision, arf, n, l2, stream=0):
 
 def rfp_potrs(precision, arf, n, b, nrhs=1, stream=0):
     return check(lib().kf_rfp_potrs(precision, arf, n, b, nrhs, stream or None), "DPFTRS")
+
+
+def rfp_symv(precision, arf, n, lam, x, y, stream=0):
+    return check(lib().kf_rfp_symv(precision, arf, n, lam, x, y, stream or None), "kf_rfp_symv")
+
+
+def krr_solve_mixed(K32, lwork, n, lam, y, alpha, tol=1e-10, max_iter=200, stream=0):
+    """Returns (iterations, relative residual, regularisation actually solved)."""
+    it, rel, mu = _c_int(0), _c_double(0.0), _c_double(0.0)
+    check(lib().kf_krr_solve_mixed(K32, lwork, n, lam, y, alpha, tol, max_iter, ctypes.byref(it),
+                                   ctypes.byref(rel), ctypes.byref(mu), stream or None),
+          "kf_krr_solve_mixed")
+    return it.value, rel.value, mu.value

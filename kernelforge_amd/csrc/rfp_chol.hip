@@ -189,145 +189,251 @@ __global__ __launch_bounds__(256) void trsm_panel(T *a, Part P, int J, const T *
     store_tile<T, false>(a, t, rn, w, Pn);
 }
 
-// Trailing update: A_IK -= A_IJ A_KJ^T for J < K <= I (one WG per lower tile pair).
+// Trailing update over a range of panels:
+//   A_IK -= sum_{J in [Jlo, Jhi)} A_IJ A_KJ^T   for K in [Kbeg, Kend), I in [K, nblk).
+// One WG per (I, K) tile.  The C tile lives in MFMA accumulators (loaded from and
+// stored to the packed array directly); panels stream through LDS with a register
+// prefetch of the next panel.  In the column-major region (rows < n/2) the WG
+// computes C^T so that consecutive lanes touch consecutive (contiguous) rows.
 template <typename T>
-__global__ __launch_bounds__(256) void syrk_update(T *a, Part P, int J, const int *info) {
+__global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi, int Kbeg,
+                                                  int Kend, const int *info) {
     __shared__ T Ai[NB * TP];
     __shared__ T Ak[NB * TP];
-    __shared__ T C[NB * TP];
     if (*info) return;
-    // decode lower-triangular pair index over the trailing blocks
-    const int m = P.nblk - J - 1;
-    const int t = blockIdx.x;
-    int ii = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((long)ii * (ii + 1) / 2 > t) --ii;
-    while ((long)(ii + 1) * (ii + 2) / 2 <= t) ++ii;
-    const int kk_ = t - ii * (ii + 1) / 2;
-    if (ii >= m) return;
-    const int I = J + 1 + ii, K = J + 1 + kk_;
-    const int c0 = P.start(J), w = P.width(J);
+    // decode t -> (K, I): K = Kbeg + q has (nblk - K) tiles I = K..nblk-1
+    const long t = blockIdx.x;
+    const long r = P.nblk - Kbeg;
+    long q = (long)((2.0 * r + 1.0 - sqrt((2.0 * r + 1.0) * (2.0 * r + 1.0) - 8.0 * (double)t)) * 0.5);
+    if (q < 0) q = 0;
+    while (q > 0 && q * r - q * (q - 1) / 2 > t) --q;
+    while ((q + 1) * r - (q + 1) * q / 2 <= t) ++q;
+    const int K = Kbeg + (int)q;
+    if (K >= Kend) return;
+    const int I = K + (int)(t - (q * r - q * (q - 1) / 2));
+    if (I >= P.nblk) return;
     const int ri = P.start(I), rni = P.width(I), rk = P.start(K), rnk = P.width(K);
-    load_tile<T, false>(a, tile_at(P, ri, c0), rni, w, Ai);
-    load_tile<T, false>(a, tile_at(P, rk, c0), rnk, w, Ak);
     const bool diagt = (I == K);
     const Tile tc = tile_at(P, ri, rk);
-    if (diagt) load_tile<T, true>(a, tc, rni, rnk, C); else load_tile<T, false>(a, tc, rni, rnk, C);
-    __syncthreads();
+    const bool trans = tc.rs == 1;  // column-major region: compute C^T
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    typename Vec4<T>::type acc[2][2];
+    typedef typename Vec4<T>::type V4;
+    V4 acc[2][2];
+    // load C (or C^T) into the accumulators
+#pragma unroll
     for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = typename Vec4<T>::type{0, 0, 0, 0};
-    for (int k0 = 0; k0 < NB; k0 += 4) {
-        const int kk = k0 + (lane >> 4);
-        T av[2], bv[2];
-        for (int i = 0; i < 2; ++i) av[i] = Ai[(wr * 32 + i * 16 + (lane & 15)) * TP + kk];
-        for (int j = 0; j < 2; ++j) bv[j] = Ak[(wc * 32 + j * 16 + (lane & 15)) * TP + kk];
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(av[i], bv[j], acc[i][j]);
-    }
-    for (int i = 0; i < 2; ++i)
+#pragma unroll
         for (int j = 0; j < 2; ++j)
-            for (int v = 0; v < 4; ++v)
-                C[(wr * 32 + i * 16 + mfma16_row<T>(lane, v)) * TP + wc * 32 + j * 16 + (lane & 15)] -= acc[i][j][v];
-    __syncthreads();
-    if (diagt) store_tile<T, true>(a, tc, rni, rnk, C); else store_tile<T, false>(a, tc, rni, rnk, C);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int x = wr * 32 + i * 16 + mfma16_row<T>(lane, v);
+                const int y = wc * 32 + j * 16 + (lane & 15);
+                const int row = trans ? y : x, col = trans ? x : y;
+                T c = T(0);
+                if (row < rni && col < rnk && (!diagt || row >= col))
+                    c = a[tc.base + (size_t)row * tc.rs + (size_t)col * tc.cs];
+                acc[i][j][v] = c;
+            }
+    // panel tiles: thread owns 16 elements of each (coalesced along the contiguous axis)
+    T pi[16], pk[16];
+    auto gload = [&](int J) {
+        const int c0 = P.start(J), w = P.width(J);
+        const Tile ti = tile_at(P, ri, c0), tk = tile_at(P, rk, c0);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int idx = threadIdx.x + 256 * e;
+            int rr, cc;
+            if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
+            pi[e] = (rr < rni && cc < w) ? a[ti.base + (size_t)rr * ti.rs + (size_t)cc * ti.cs] : T(0);
+            if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
+            pk[e] = (rr < rnk && cc < w) ? a[tk.base + (size_t)rr * tk.rs + (size_t)cc * tk.cs] : T(0);
+        }
+    };
+    auto sstore = [&](int J) {
+        const int c0 = P.start(J);
+        const Tile ti = tile_at(P, ri, c0), tk = tile_at(P, rk, c0);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int idx = threadIdx.x + 256 * e;
+            int rr, cc;
+            if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
+            Ai[rr * TP + cc] = pi[e];
+            if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
+            Ak[rr * TP + cc] = pk[e];
+        }
+    };
+    gload(Jlo);
+    for (int J = Jlo; J < Jhi; ++J) {
+        __syncthreads();
+        sstore(J);
+        __syncthreads();
+        if (J + 1 < Jhi) gload(J + 1);
+        const T *X = trans ? Ak : Ai;  // rows of the computed tile
+        const T *Y = trans ? Ai : Ak;  // columns
+#pragma unroll 4
+        for (int k0 = 0; k0 < NB; k0 += 4) {
+            const int kk = k0 + (lane >> 4);
+            T av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = X[(wr * 32 + i * 16 + (lane & 15)) * TP + kk];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Y[(wc * 32 + j * 16 + (lane & 15)) * TP + kk];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(-av[i], bv[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int x = wr * 32 + i * 16 + mfma16_row<T>(lane, v);
+                const int y = wc * 32 + j * 16 + (lane & 15);
+                const int row = trans ? y : x, col = trans ? x : y;
+                if (row < rni && col < rnk && (!diagt || row >= col))
+                    a[tc.base + (size_t)row * tc.rs + (size_t)col * tc.cs] = acc[i][j][v];
+            }
+}
+
+inline long range_tiles(const Part &P, int Kbeg, int Kend) {
+    long n = 0;
+    for (int K = Kbeg; K < Kend; ++K) n += P.nblk - K;
+    return n;
 }
 
 // ---------------------------------------------------------------------------
 // Triangular solves, FP64 accumulation.  b is (nrhs, n) row-major.
+// Diagonal blocks are inverted once per solve (all in parallel); each block step is
+// then one launch: every WG forms y_J = Linv_J b_J (64x64), WG 0 stores it, the
+// others apply it to their block.
 // ---------------------------------------------------------------------------
-template <typename T, bool TRANS>
-__global__ __launch_bounds__(64) void diag_solve(const T *a, Part P, int J, double *b, int n, int nrhs) {
+template <typename T>
+__global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *inv) {
     __shared__ double L[NB * TP];
-    __shared__ double y[NB];
+    __shared__ double Li[NB * TP];
+    const int J = blockIdx.x;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
         double v = 0.0;
-        if (r < w && c < w && r >= c) v = (double)a[t.base + r * t.rs + c * t.cs];
+        if (r < w && c < w && r >= c) v = (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs];
         L[r * TP + c] = v;
+        Li[r * TP + c] = 0.0;
     }
     __syncthreads();
-    const int lane = threadIdx.x;
-    for (int q = 0; q < nrhs; ++q) {
-        double *bq = b + (size_t)q * n + s0;
-        if (lane < w) y[lane] = bq[lane];
-        __syncthreads();
-        if (!TRANS) {  // L y = b, forward
-            for (int j = 0; j < w; ++j) {
-                double part = lane < j ? L[j * TP + lane] * y[lane] : 0.0;
-                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-                if (lane == 0) y[j] = (y[j] - part) / L[j * TP + j];
-                __syncthreads();
-            }
-        } else {  // L^T x = y, backward
-            for (int j = w - 1; j >= 0; --j) {
-                double part = (lane > j && lane < w) ? L[lane * TP + j] * y[lane] : 0.0;
-                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-                if (lane == 0) y[j] = (y[j] - part) / L[j * TP + j];
-                __syncthreads();
-            }
+    if (threadIdx.x < w) {
+        const int c = threadIdx.x;
+        for (int r = c; r < w; ++r) {
+            double acc = (r == c) ? 1.0 : 0.0;
+            for (int q = c; q < r; ++q) acc -= L[r * TP + q] * Li[q * TP + c];
+            Li[r * TP + c] = acc / L[r * TP + r];
         }
-        if (lane < w) bq[lane] = y[lane];
-        __syncthreads();
     }
+    __syncthreads();
+    double *dst = inv + (size_t)J * NB * NB;
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x)
+        dst[idx] = Li[(idx / NB) * TP + idx % NB];
 }
 
-// forward update: b_I -= L_IJ y_J for blocks I > J (one WG per I, 64 threads = rows)
+// forward step J: y_J = Linv_J b_J (stored to y by WG 0); b_I -= L_IJ y_J for I > J.
 template <typename T>
-__global__ __launch_bounds__(256) void fwd_update(const T *a, Part P, int J, double *b, int n, int nrhs) {
+__global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const double *inv,
+                                                double *b, double *y, int n, int nrhs) {
     __shared__ double yj[NB];
     __shared__ double Lt[NB * TP];
-    const int I = J + 1 + blockIdx.x;
-    const int r0 = P.start(I), rn = P.width(I), c0 = P.start(J), w = P.width(J);
-    const Tile t = tile_at(P, r0, c0);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-        int r, c;
-        if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        Lt[r * TP + c] = (r < rn && c < w) ? (double)a[t.base + r * t.rs + c * t.cs] : 0.0;
+    const int I = J + blockIdx.x;  // blockIdx 0 -> the diagonal block itself
+    const int c0 = P.start(J), w = P.width(J);
+    const double *Li = inv + (size_t)J * NB * NB;
+    int r0 = 0, rn = 0;
+    if (blockIdx.x > 0) {
+        r0 = P.start(I);
+        rn = P.width(I);
+        const Tile t = tile_at(P, r0, c0);
+        for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+            int r, c;
+            if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
+            Lt[r * TP + c] = (r < rn && c < w) ? (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs] : 0.0;
+        }
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
-        if (threadIdx.x < w) yj[threadIdx.x] = b[(size_t)q * n + c0 + threadIdx.x];
+        // y_J = Linv_J b_J: 4 threads per row
+        {
+            const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            double s = 0.0;
+            if (r < w)
+                for (int c = qq; c < w; c += 4) s += Li[r * NB + c] * b[(size_t)q * n + c0 + c];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (qq == 0 && r < NB) yj[r] = r < w ? s : 0.0;
+        }
         __syncthreads();
-        // 4 threads per row, each a quarter of the columns
-        const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
-        double s = 0.0;
-        for (int c = qq; c < w; c += 4) s += Lt[r * TP + c] * yj[c];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] -= s;
+        if (blockIdx.x == 0) {
+            if (threadIdx.x < w) y[(size_t)q * n + c0 + threadIdx.x] = yj[threadIdx.x];
+        } else {
+            const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            double s = 0.0;
+            for (int c = qq; c < w; c += 4) s += Lt[r * TP + c] * yj[c];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] -= s;
+        }
     }
 }
 
-// backward update: x_c -= sum_{r in J} L[r][c] x_r for columns c in block I < J
+// backward step J: x_J = Linv_J^T y_J (stored to x by WG 0); y_I -= L_JI^T x_J for I < J.
 template <typename T>
-__global__ __launch_bounds__(256) void bwd_update(const T *a, Part P, int J, double *b, int n, int nrhs) {
+__global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const double *inv,
+                                                double *y, double *x, int n, int nrhs) {
     __shared__ double xj[NB];
     __shared__ double Lt[NB * TP];
-    const int I = blockIdx.x;  // I < J
-    const int r0 = P.start(J), w = P.width(J), c0 = P.start(I), cn = P.width(I);
-    const Tile t = tile_at(P, r0, c0);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-        int r, c;
-        if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        Lt[r * TP + c] = (r < w && c < cn) ? (double)a[t.base + r * t.rs + c * t.cs] : 0.0;
+    const int r0 = P.start(J), w = P.width(J);
+    const double *Li = inv + (size_t)J * NB * NB;
+    const int I = (int)blockIdx.x - 1;  // blockIdx 0 -> the diagonal block
+    int c0 = 0, cn = 0;
+    if (blockIdx.x > 0) {
+        c0 = P.start(I);
+        cn = P.width(I);
+        const Tile t = tile_at(P, r0, c0);
+        for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+            int r, c;
+            if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
+            Lt[r * TP + c] = (r < w && c < cn) ? (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs] : 0.0;
+        }
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
-        if (threadIdx.x < w) xj[threadIdx.x] = b[(size_t)q * n + r0 + threadIdx.x];
+        {
+            // x_J[c] = sum_r Linv[r][c] y_J[r]
+            const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            double s = 0.0;
+            if (c < w)
+                for (int r = qq; r < w; r += 4) s += Li[r * NB + c] * y[(size_t)q * n + r0 + r];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (qq == 0 && c < NB) xj[c] = c < w ? s : 0.0;
+        }
         __syncthreads();
-        const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
-        double s = 0.0;
-        for (int r = qq; r < w; r += 4) s += Lt[r * TP + c] * xj[r];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (qq == 0 && c < cn) b[(size_t)q * n + c0 + c] -= s;
+        if (blockIdx.x == 0) {
+            if (threadIdx.x < w) x[(size_t)q * n + r0 + threadIdx.x] = xj[threadIdx.x];
+        } else {
+            const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            double s = 0.0;
+            for (int r = qq; r < w; r += 4) s += Lt[r * TP + c] * xj[r];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (qq == 0 && c < cn) y[(size_t)q * n + c0 + c] -= s;
+        }
     }
 }
+
+constexpr int NBO = 4;  // 64-wide panels per outer (256-deep) trailing update
 
 template <typename T>
 int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
@@ -342,13 +448,23 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     if (l2 != 0.0) {
         hipLaunchKernelGGL(add_diag<T>, dim3((n + 255) / 256), dim3(256), 0, stream, a, n, l2);
     }
-    for (int J = 0; J < P.nblk; ++J) {
-        hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, stream, a, P, J, inv, info);
-        const int m = P.nblk - J - 1;
-        if (m > 0) {
-            hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, stream, a, P, J, inv, info);
-            hipLaunchKernelGGL(syrk_update<T>, dim3(m * (m + 1) / 2), dim3(256), 0, stream, a, P,
-                               J, info);
+    for (int J0 = 0; J0 < P.nblk; J0 += NBO) {
+        const int J1 = J0 + NBO < P.nblk ? J0 + NBO : P.nblk;
+        for (int J = J0; J < J1; ++J) {
+            hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, stream, a, P, J, inv, info);
+            const int m = P.nblk - J - 1;
+            if (m > 0)
+                hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, stream, a, P, J, inv, info);
+            if (J + 1 < J1) {  // remaining columns of this outer panel
+                const long nt = range_tiles(P, J + 1, J1);
+                hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P,
+                                   J, J + 1, J + 1, J1, info);
+            }
+        }
+        if (J1 < P.nblk) {  // everything right of the outer panel, 256-deep
+            const long nt = range_tiles(P, J1, P.nblk);
+            hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, J0,
+                               J1, J1, P.nblk, info);
         }
     }
     KF_HIP_TRY(hipGetLastError());
@@ -363,18 +479,20 @@ template <typename T>
 int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
     KF_REQUIRE(a && b && n > 0 && nrhs >= 1, KF_ERR_ARG, "bad potrs arguments");
     const Part P = make_part(n);
+    const size_t inv_bytes = sizeof(double) * (size_t)P.nblk * NB * NB;
+    char *ws = static_cast<char *>(
+        workspace(WS_AUX3, inv_bytes + sizeof(double) * (size_t)n * nrhs + 256, stream));
+    KF_REQUIRE(ws, KF_ERR_NOMEM, "workspace allocation failed");
+    double *inv = reinterpret_cast<double *>(ws);
+    double *y = reinterpret_cast<double *>(ws + inv_bytes);
     ScopedTimer timer(TM_POTRS, stream);
-    for (int J = 0; J < P.nblk; ++J) {
-        hipLaunchKernelGGL((diag_solve<T, false>), dim3(1), dim3(64), 0, stream, a, P, J, b, n, nrhs);
-        const int m = P.nblk - J - 1;
-        if (m > 0)
-            hipLaunchKernelGGL(fwd_update<T>, dim3(m), dim3(256), 0, stream, a, P, J, b, n, nrhs);
-    }
-    for (int J = P.nblk - 1; J >= 0; --J) {
-        hipLaunchKernelGGL((diag_solve<T, true>), dim3(1), dim3(64), 0, stream, a, P, J, b, n, nrhs);
-        if (J > 0)
-            hipLaunchKernelGGL(bwd_update<T>, dim3(J), dim3(256), 0, stream, a, P, J, b, n, nrhs);
-    }
+    hipLaunchKernelGGL(inv_diag_all<T>, dim3(P.nblk), dim3(256), 0, stream, a, P, inv);
+    for (int J = 0; J < P.nblk; ++J)
+        hipLaunchKernelGGL(fwd_step<T>, dim3(P.nblk - J), dim3(256), 0, stream, a, P, J, inv, b,
+                           y, n, nrhs);
+    for (int J = P.nblk - 1; J >= 0; --J)
+        hipLaunchKernelGGL(bwd_step<T>, dim3(J + 1), dim3(256), 0, stream, a, P, J, inv, y, b, n,
+                           nrhs);
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }

@@ -103,6 +103,7 @@ def test_bf16x3_symm_config0(lk, config0):
 def test_bf16x3_rect_and_random(lk, config0):
     _, _, _, C, Q, N = config0
     X = oracle.fchl19_batch(C[:300], Q[:300], N[:300])
+    Q, N = Q[:300], N[:300]
     Kr = lk.kernel_gaussian(X[:70], X[70:], Q[:70], Q[70:], N[:70], N[70:], 2.0, precision="bf16x3")
     assert rel_err(Kr, oracle.kernel_rect(X[:70], X[70:], Q[:70], Q[70:], N[:70], N[70:], 2.0)) < 2e-5
     x, q, n = random_set(150, 6, 33, 3, 5)

@@ -1,0 +1,84 @@
+"""GPU: RFP symmetric matvec, PCG and the mixed-precision KRR solve (kf_rfp_symv,
+kf_rfp_pcg, kf_krr_solve_mixed) vs NumPy / the FP64 oracle."""
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def spd(n, seed, shift=1.0):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n))
+    return A @ A.T / n + shift * np.eye(n)
+
+
+def to_rfp(K):
+    trttf = sl.get_lapack_funcs("trttf", (K,))
+    arf, info = trttf(np.asfortranarray(K), transr="N", uplo="U")
+    return arf
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 130, 257, 1000])
+def test_symv_f64_f32(n):
+    from kernelforge_amd import capi
+
+    K = spd(n, n)
+    x = np.random.default_rng(1).standard_normal(n)
+    DA = capi.DeviceArray
+    a64, a32, dx, dy = DA.from_numpy(to_rfp(K)), DA.from_numpy(to_rfp(K).astype(np.float32)), DA.from_numpy(x), DA(n)
+    capi.rfp_symv(capi.KF_PREC_F64, a64.ptr, n, 0.25, dx.ptr, dy.ptr)
+    capi.synchronize()
+    assert rel_err(dy.numpy(), (K + 0.25 * np.eye(n)) @ x) < 1e-13
+    capi.rfp_symv(capi.KF_PREC_F32, a32.ptr, n, 0.0, dx.ptr, dy.ptr)
+    capi.synchronize()
+    K32 = K.astype(np.float32).astype(np.float64)
+    assert rel_err(dy.numpy(), K32 @ x) < 1e-12
+
+
+def test_pcg_with_exact_factor_converges_immediately():
+    from kernelforge_amd import capi
+    import ctypes
+
+    n = 300
+    K = spd(n, 3)
+    y = np.random.default_rng(4).standard_normal(n)
+    DA = capi.DeviceArray
+    A = DA.from_numpy(to_rfp(K))
+    L = DA.from_numpy(to_rfp(K))
+    capi.rfp_potrf(capi.KF_PREC_F64, L.ptr, n, 1e-3)
+    dy, da = DA.from_numpy(y), DA(n)
+    da.zero()
+    it, rel = ctypes.c_int(0), ctypes.c_double(0)
+    capi.check(capi.lib().kf_rfp_pcg(capi.KF_PREC_F64, A.ptr, capi.KF_PREC_F64, L.ptr, n, 1e-3,
+                                     dy.ptr, da.ptr, 1e-12, 10, ctypes.byref(it), ctypes.byref(rel),
+                                     None), "pcg")
+    assert it.value <= 2 and rel.value < 1e-12
+    assert rel_err(da.numpy(), np.linalg.solve(K + 1e-3 * np.eye(n), y)) < 1e-10
+
+
+def test_mixed_solve_reaches_fp64_residual(config0):
+    """BF16x3 kernel -> FP32 factor -> PCG: the FP64 residual of the FP32 system reaches
+    1e-10; alpha / predictions vs the FP64 oracle are reported in bench.py."""
+    from kernelforge_amd import capi, local_kernels
+
+    _, _, energies, C, Q, N = config0
+    X = oracle.fchl19_batch(C[:800], Q[:800], N[:800])
+    y = energies[:800] - energies[:800].mean()
+    arf32 = local_kernels.kernel_gaussian_symm_rfp(X, Q[:800], N[:800], 2.0, precision="bf16x3")
+    n = 800
+    DA = capi.DeviceArray
+    K32, Lw, dy, da = DA.from_numpy(arf32), DA(n * (n + 1) // 2, np.float32), DA.from_numpy(y), DA(n)
+    it, rel, mu = capi.krr_solve_mixed(K32.ptr, Lw.ptr, n, 1e-8, dy.ptr, da.ptr, tol=1e-10,
+                                       max_iter=200)
+    assert rel <= 1e-10
+    # the residual is that of the FP32-stored system: check it on the host in FP64
+    Kf = sl.get_lapack_funcs("tfttr", (arf32.astype(np.float64),))(n, arf32.astype(np.float64),
+                                                                   transr="N", uplo="U")[0]
+    Kf = np.triu(Kf) + np.triu(Kf, 1).T
+    a = da.numpy()
+    assert np.linalg.norm((Kf + 1e-8 * np.eye(n)) @ a - y) / np.linalg.norm(y) < 1e-9

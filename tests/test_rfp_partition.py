@@ -46,7 +46,27 @@ def rfp_lower(n, r, c):
     return (k + 1 + r) + c * s if r < k else (r - k) * s + c
 
 
+NBO = 4  # 64-wide panels per outer trailing update
+
+
+def range_decode(t, nblk, Kbeg):
+    """Tile index t -> (K, I) exactly as syrk_range decodes it."""
+    import math
+    r = nblk - Kbeg
+    q = int((2.0 * r + 1.0 - math.sqrt((2.0 * r + 1.0) ** 2 - 8.0 * t)) * 0.5)
+    q = max(q, 0)
+    while q > 0 and q * r - q * (q - 1) // 2 > t:
+        q -= 1
+    while (q + 1) * r - (q + 1) * q // 2 <= t:
+        q += 1
+    K = Kbeg + q
+    return K, K + (t - (q * r - q * (q - 1) // 2))
+
+
 def blocked_potrf(arf, n):
+    """Replay of potrf_impl: per 256-column outer panel, 64-wide inner steps
+    (diag POTRF, panel TRSM, syrk_range over the rest of the outer panel), then one
+    256-deep syrk_range over everything to its right."""
     k, nblk, start, width = part(n)
     a = arf.copy()
     size = n * (n + 1) // 2
@@ -70,27 +90,41 @@ def blocked_potrf(arf, n):
                     continue
                 a[tile_addr(n, r0, c0, r, c)] = T[r, c]
 
-    for J in range(nblk):
-        s0, w = start(J), width(J)
-        L = load(s0, w, s0, w, True)[:w, :w]
-        L = np.linalg.cholesky(np.tril(L) + np.tril(L, -1).T)
-        inv = np.linalg.inv(L)
-        T = np.zeros((NB, NB))
-        T[:w, :w] = L
-        store(T, s0, w, s0, w, True)
-        for I in range(J + 1, nblk):
-            r0, rn = start(I), width(I)
-            P = load(r0, rn, s0, w, False)
-            P[:rn, :w] = P[:rn, :w] @ inv.T
-            store(P, r0, rn, s0, w, False)
-        for I in range(J + 1, nblk):
-            for K in range(J + 1, I + 1):
-                ri, rni, rk, rnk = start(I), width(I), start(K), width(K)
-                Ai = load(ri, rni, s0, w, False)
-                Ak = load(rk, rnk, s0, w, False)
-                C = load(ri, rni, rk, rnk, I == K)
+    def syrk_range(Jlo, Jhi, Kbeg, Kend):
+        nt = sum(nblk - K for K in range(Kbeg, Kend))
+        seen = set()
+        for t in range(nt):
+            K, I = range_decode(t, nblk, Kbeg)
+            assert Kbeg <= K < Kend and K <= I < nblk
+            seen.add((I, K))
+            ri, rni, rk, rnk = start(I), width(I), start(K), width(K)
+            C = load(ri, rni, rk, rnk, I == K)
+            for J in range(Jlo, Jhi):
+                Ai = load(ri, rni, start(J), width(J), False)
+                Ak = load(rk, rnk, start(J), width(J), False)
                 C -= Ai @ Ak.T
-                store(C, ri, rni, rk, rnk, I == K)
+            store(C, ri, rni, rk, rnk, I == K)
+        assert len(seen) == nt
+
+    for J0 in range(0, nblk, NBO):
+        J1 = min(J0 + NBO, nblk)
+        for J in range(J0, J1):
+            s0, w = start(J), width(J)
+            L = load(s0, w, s0, w, True)[:w, :w]
+            L = np.linalg.cholesky(np.tril(L) + np.tril(L, -1).T)
+            inv = np.linalg.inv(L)
+            T = np.zeros((NB, NB))
+            T[:w, :w] = L
+            store(T, s0, w, s0, w, True)
+            for I in range(J + 1, nblk):
+                r0, rn = start(I), width(I)
+                Pn = load(r0, rn, s0, w, False)
+                Pn[:rn, :w] = Pn[:rn, :w] @ inv.T
+                store(Pn, r0, rn, s0, w, False)
+            if J + 1 < J1:
+                syrk_range(J, J + 1, J + 1, J1)
+        if J1 < nblk:
+            syrk_range(J0, J1, J1, nblk)
     return a
 
 
@@ -106,7 +140,7 @@ def test_partition_covers_and_aligns(n):
     assert cols == list(range(n))
 
 
-@pytest.mark.parametrize("n", [3, 64, 130, 203])
+@pytest.mark.parametrize("n", [3, 64, 130, 203, 390])
 def test_blocked_rfp_cholesky_matches_lapack(n):
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
