@@ -1,4 +1,5 @@
 // kf_runtime.hip -- error state, device workspace cache, library info.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <string>
@@ -72,6 +73,25 @@ void *host_staging(int slot, size_t bytes) {
     s.ptr = p;
     s.bytes = bytes;
     return p;
+}
+
+// Copy Q (nm x ma) and N (nm) from the device to host vectors (synchronises the stream).
+int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &hQ,
+                  std::vector<int> &hN, hipStream_t stream, int slot) {
+    hQ.resize((size_t)nm * ma);
+    hN.resize(nm);
+    int *stage = static_cast<int *>(host_staging(slot, sizeof(int) * ((size_t)nm * ma + nm)));
+    KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
+    if (nm * (size_t)ma)
+        KF_HIP_TRY(hipMemcpyAsync(stage, Q, sizeof(int) * (size_t)nm * ma, hipMemcpyDeviceToHost,
+                                  stream));
+    if (nm)
+        KF_HIP_TRY(hipMemcpyAsync(stage + (size_t)nm * ma, N, sizeof(int) * nm,
+                                  hipMemcpyDeviceToHost, stream));
+    KF_HIP_TRY(hipStreamSynchronize(stream));
+    std::copy(stage, stage + (size_t)nm * ma, hQ.begin());
+    std::copy(stage + (size_t)nm * ma, stage + (size_t)nm * ma + nm, hN.begin());
+    return KF_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -248,23 +248,6 @@ struct DevicePacked {
     int rows = 0, rep_pad = 0;
 };
 
-int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &hQ,
-                  std::vector<int> &hN, hipStream_t stream, int slot) {
-    hQ.resize((size_t)nm * ma);
-    hN.resize(nm);
-    int *stage = static_cast<int *>(host_staging(slot, sizeof(int) * ((size_t)nm * ma + nm)));
-    KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
-    if (nm * (size_t)ma)
-        KF_HIP_TRY(hipMemcpyAsync(stage, Q, sizeof(int) * (size_t)nm * ma, hipMemcpyDeviceToHost,
-                                  stream));
-    KF_HIP_TRY(hipMemcpyAsync(stage + (size_t)nm * ma, N, sizeof(int) * nm, hipMemcpyDeviceToHost,
-                              stream));
-    KF_HIP_TRY(hipStreamSynchronize(stream));
-    std::copy(stage, stage + (size_t)nm * ma, hQ.begin());
-    std::copy(stage + (size_t)nm * ma, stage + (size_t)nm * ma + nm, hN.begin());
-    return KF_OK;
-}
-
 int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, int ws_plan,
                     hipStream_t stream, DevicePacked &d) {
     d.rows = p.rows;

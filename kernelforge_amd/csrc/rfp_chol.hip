@@ -103,6 +103,26 @@ __global__ void add_diag(T *a, int n, double l2) {
     if (j < n) a[rfp_lower((size_t)n, j, j)] += T(l2);
 }
 
+// Inverse of a lower-triangular w x w LDS tile L (pitch TP) into Li (zeroed by the
+// caller), 256 threads: rows advance in sequence, the 64 columns of a row in
+// parallel, each dot product split over 4 threads.
+template <typename T, typename U>
+__device__ inline void tri_inverse(const T *L, U *Li, U *red, int w) {
+    const int c = threadIdx.x & 63, part = threadIdx.x >> 6;
+    for (int r = 0; r < w; ++r) {
+        U s = U(0);
+        if (c < r)
+            for (int q = c + part; q < r; q += 4) s += (U)L[r * TP + q] * Li[q * TP + c];
+        red[part * 64 + c] = s;
+        __syncthreads();
+        if (part == 0 && c <= r) {
+            const U tot = red[c] + red[64 + c] + red[128 + c] + red[192 + c];
+            Li[r * TP + c] = ((r == c ? U(1) : U(0)) - tot) / (U)L[r * TP + r];
+        }
+        __syncthreads();
+    }
+}
+
 // POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB].
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
@@ -138,18 +158,11 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
         if (threadIdx.x == 0) atomicCAS(info, 0, fail);
         return;
     }
-    // inverse of the lower factor: thread c owns column c
+    // inverse of the lower factor
+    __shared__ T red[256];
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) Li[idx / NB * TP + idx % NB] = T(0);
     __syncthreads();
-    if (threadIdx.x < w) {
-        const int c = threadIdx.x;
-        for (int r = c; r < w; ++r) {
-            T acc = (r == c) ? T(1) : T(0);
-            for (int q = c; q < r; ++q) acc -= L[r * TP + q] * Li[q * TP + c];
-            Li[r * TP + c] = acc / L[r * TP + r];
-        }
-    }
-    __syncthreads();
+    tri_inverse<T, T>(L, Li, red, w);
     store_tile<T, true>(a, t, w, w, L);
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) inv[idx] = Li[(idx / NB) * TP + idx % NB];
 }
@@ -326,16 +339,9 @@ __global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *
         L[r * TP + c] = v;
         Li[r * TP + c] = 0.0;
     }
+    __shared__ double red[256];
     __syncthreads();
-    if (threadIdx.x < w) {
-        const int c = threadIdx.x;
-        for (int r = c; r < w; ++r) {
-            double acc = (r == c) ? 1.0 : 0.0;
-            for (int q = c; q < r; ++q) acc -= L[r * TP + q] * Li[q * TP + c];
-            Li[r * TP + c] = acc / L[r * TP + r];
-        }
-    }
-    __syncthreads();
+    tri_inverse<double, double>(L, Li, red, w);
     double *dst = inv + (size_t)J * NB * NB;
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x)
         dst[idx] = Li[(idx / NB) * TP + idx % NB];
