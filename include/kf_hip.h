@@ -29,6 +29,7 @@
 #define KF_HIP_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -116,6 +117,162 @@ int kf_local_kernel_symm(int precision, const double *X, const int *Q, const int
 int kf_local_kernel(int precision, const double *X1, const double *X2, const int *Q1,
                     const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
                     int max_atoms1, int max_atoms2, int rep, double sigma, void *K, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Energy+force forms.  dX arrays are (nm, max_atoms, rep, 3*max_atoms): the  */
+/* Jacobian of each atom's descriptor with respect to its molecule's         */
+/* coordinates; naq = 3 * sum(clamped N).  max_atoms <= 32 for these forms   */
+/* (KF_ERR_BASIS otherwise).  All arrays are device pointers.                */
+/* ------------------------------------------------------------------------ */
+
+/* (nm1, naq2) row-major, +dK/dR2.  Replaces kf::fchl19::kernel_gaussian_jacobian
+ * (src/local_kernels.hpp:26-42, src/local_kernels.cpp:791-974). */
+int kf_local_kernel_jacobian(const double *X1, const double *X2, const double *dX2,
+                             const int *Q1, const int *Q2, const int *N1, const int *N2, int nm1,
+                             int nm2, int max_atoms1, int max_atoms2, int rep, double sigma,
+                             double *K, void *stream);
+
+/* (naq1, nm2) row-major, -dK/dR1.  Replaces kernel_gaussian_jacobian_t
+ * (src/local_kernels.hpp:44-56, src/local_kernels.cpp:991-1166). */
+int kf_local_kernel_jacobian_t(const double *X1, const double *X2, const double *dX1,
+                               const int *Q1, const int *Q2, const int *N1, const int *N2,
+                               int nm1, int nm2, int max_atoms1, int max_atoms2, int rep,
+                               double sigma, double *K, void *stream);
+
+/* (naq1, naq2) row-major Hessian kernel.  Replaces kernel_gaussian_hessian
+ * (src/local_kernels.hpp:58-72, src/local_kernels.cpp:1171-1419). */
+int kf_local_kernel_hessian(const double *X1, const double *X2, const double *dX1,
+                            const double *dX2, const int *Q1, const int *Q2, const int *N1,
+                            const int *N2, int nm1, int nm2, int max_atoms1, int max_atoms2,
+                            int rep, double sigma, double *K, void *stream);
+
+/* (nm1+naq1, nm2+naq2) row-major [[EE, EF], [FE, FF]].  Replaces
+ * kernel_gaussian_full (src/local_kernels.hpp:99-107, src/local_kernels.cpp:1918-2195). */
+int kf_local_kernel_full(const double *X1, const double *X2, const double *dX1,
+                         const double *dX2, const int *Q1, const int *Q2, const int *N1,
+                         const int *N2, int nm1, int nm2, int max_atoms1, int max_atoms2,
+                         int rep, double sigma, double *K, void *stream);
+
+/* (naq, naq) dense symmetric Hessian kernel.  Replaces kernel_gaussian_hessian_symm
+ * (src/local_kernels.hpp:74-83, src/local_kernels.cpp:1421-1662). */
+int kf_local_kernel_hessian_symm(const double *X, const double *dX, const int *Q, const int *N,
+                                 int nm, int max_atoms, int rep, double sigma, double *K,
+                                 void *stream);
+
+/* Hessian kernel in RFP (TRANSR='N', UPLO='U'), naq(naq+1)/2 doubles.  Replaces
+ * kernel_gaussian_hessian_symm_rfp (src/local_kernels.hpp:85-97, :1664-1900). */
+int kf_local_kernel_hessian_symm_rfp(const double *X, const double *dX, const int *Q,
+                                     const int *N, int nm, int max_atoms, int rep,
+                                     double sigma, double *arf, void *stream);
+
+/* (nm+naq)^2 dense symmetric full kernel.  Replaces kernel_gaussian_full_symm
+ * (src/local_kernels.hpp:109-115, src/local_kernels.cpp:2201-2505). */
+int kf_local_kernel_full_symm(const double *X, const double *dX, const int *Q, const int *N,
+                              int nm, int max_atoms, int rep, double sigma, double *K,
+                              void *stream);
+
+/* Full kernel in RFP (TRANSR='N', UPLO='U') of order BIG = nm + naq.  Replaces
+ * kernel_gaussian_full_symm_rfp (src/local_kernels.hpp:117-121, :2511-2815). */
+int kf_local_kernel_full_symm_rfp(const double *X, const double *dX, const int *Q,
+                                  const int *N, int nm, int max_atoms, int rep, double sigma,
+                                  double *arf, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Matrix-free products with the energy+force kernel (the PCG operator).     */
+/* ------------------------------------------------------------------------ */
+
+/* alpha_desc[b,j,:] = dX[b,j,:,:3N_b] @ alpha[offs_b : offs_b + 3N_b], zero rows for
+ * j >= N_b; out is (nm, max_atoms, rep).  Replaces
+ * kernel_gaussian_local_compute_alpha_desc (src/local_kernels.hpp:141-169,
+ * src/local_kernels.cpp:2821-2893).  Q is accepted for signature parity. */
+int kf_local_alpha_desc(const double *dX, const int *Q, const int *N, int nm, int max_atoms,
+                        int rep, const double *alpha, double *out, void *stream);
+
+/* (E (nm1,), F (naq1,)) = K_full(set1, set2) @ [alpha_E; alpha_F] with alpha_desc from
+ * kf_local_alpha_desc(dX2, .., alpha_F).  Replaces kernel_gaussian_local_full_matvec
+ * (src/local_kernels.hpp:214-229, src/local_kernels.cpp:3263-3425). */
+int kf_local_full_matvec(const double *X1, const double *dX1, const double *X2,
+                         const double *alpha_desc, const double *alpha_E, const int *Q1,
+                         const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
+                         int max_atoms1, int max_atoms2, int rep, double sigma, double *E,
+                         double *F, void *stream);
+
+/* F (naq1,) = Hessian kernel @ alpha_F.  Replaces kernel_gaussian_local_hessian_matvec
+ * (src/local_kernels.hpp:171-191, src/local_kernels.cpp:2895-3145). */
+int kf_local_hessian_matvec(const double *X1, const double *dX1, const double *X2,
+                            const double *alpha_desc, const int *Q1, const int *Q2,
+                            const int *N1, const int *N2, int nm1, int nm2, int max_atoms1,
+                            int max_atoms2, int rep, double sigma, double *F, void *stream);
+
+/* E (nm1,) = Jacobian kernel @ alpha_F.  Replaces kernel_gaussian_local_jacobian_t_matvec
+ * (src/local_kernels.hpp:193-212, src/local_kernels.cpp:3155-3253). */
+int kf_local_jacobian_t_matvec(const double *X1, const double *X2, const double *alpha_desc,
+                               const int *Q1, const int *Q2, const int *N1, const int *N2,
+                               int nm1, int nm2, int max_atoms1, int max_atoms2, int rep,
+                               double sigma, double *E, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Energy+force solve by PCG with a (learned) block-Jacobi preconditioner.    */
+/* The E+F vector order is [E_0..E_{nm-1}, F_0 (3 N_0), F_1, ...]; block b    */
+/* holds E_b then F_b, bs = 1 + 3 max_atoms.  These have no counterpart in    */
+/* the reference (which solves E+F systems only by RFP Cholesky,             */
+/* src/math.cpp:134-171); they serve the matrix-free solve of that system.   */
+/* ------------------------------------------------------------------------ */
+
+/* blocks (nm, bs, bs): the diagonal (E_b, F_b) blocks of the symmetric full kernel
+ * (the entries kf_local_kernel_full_symm places at those rows / columns); entries
+ * beyond 1 + 3 N_b are left untouched. */
+int kf_local_full_diag_blocks(const double *X, const double *dX, const int *Q, const int *N,
+                              int nm, int max_atoms, int rep, double sigma, double *blocks,
+                              void *stream);
+
+/* inv_b = (blocks_b + lambda I + diag(shift_b))^-1 on the live 1 + 3 N_b rows (identity on
+ * padded rows); shift (nb, bs) may be null.  info (nb ints, may be null) gets 0 or the
+ * 1-based pivot at which the block was not positive definite. */
+int kf_block_factor(const double *blocks, int nb, int bs, const int *N, int max_atoms,
+                    double lambda, const double *shift, double *inv, int *info, void *stream);
+
+/* z = blockdiag(inv) r in the E+F vector order. */
+int kf_block_apply(const double *inv, int nb, int bs, const int *N, int max_atoms,
+                   const double *r, double *z, void *stream);
+
+/* PCG on (A + lambda I) x = y, A an explicit RFP (TRANSR='N', UPLO='U') E+F kernel of
+ * order n = nb + 3 sum N; inv from kf_block_factor, or null for plain CG.  Stops at
+ * ||r||/||y|| <= tol or max_iter; *rel is the true final residual; history (max_iter+1
+ * doubles, may be null) receives the recurrence residual per iteration. */
+int kf_ef_pcg_rfp(const double *arf, int n, double lambda, const double *inv, int nb, int bs,
+                  const int *N, int max_atoms, const double *y, double *x, double tol,
+                  int max_iter, int *iters, double *rel, double *history, void *stream);
+
+/* The same with the matrix-free operator K_full x = full_matvec(alpha_desc(x_F), x_E)
+ * of the training set (X, dX). */
+int kf_ef_pcg_matfree(const double *X, const double *dX, const int *Q, const int *N, int nm,
+                      int max_atoms, int rep, double sigma, double lambda, const double *inv,
+                      int bs, const double *y, double *x, double tol, int max_iter, int *iters,
+                      double *rel, double *history, void *stream);
+
+/* loss = log(||r_steps|| / ||y||) after `steps` PCG iterations from x = 0 with the RFP
+ * operator and inv = (K_bb + lambda + diag(s))^-1; adds d loss / d s to grad_shift
+ * (nb, bs) by reverse-mode differentiation of the recurrences. */
+int kf_pcg_unrolled_grad_rfp(const double *arf, int n, double lambda, const double *inv, int nb,
+                             int bs, const int *N, int max_atoms, const double *y, int steps,
+                             double *loss, double *grad_shift, void *stream);
+
+/* Preconditioner MLP (BF16 weights W (fan_in x fan_out) as uint16, FP32 everything
+ * else): Y = relu(relu(X W1 + b1) W2 + b2) W3 + b3, keeping Z1, Z2 (B x H) for the
+ * backward.  No counterpart in the reference. */
+int kf_mlp_forward(const float *X, int B, int D0, int H, int D3, const uint16_t *W1,
+                   const float *b1, const uint16_t *W2, const float *b2, const uint16_t *W3,
+                   const float *b3, float *Z1, float *Z2, float *Y, void *stream);
+int kf_mlp_backward(const float *X, int B, int D0, int H, int D3, const uint16_t *W1,
+                    const uint16_t *W2, const uint16_t *W3, const float *Z1, const float *Z2,
+                    const float *dY, float *dW1, float *db1, float *dW2, float *db2,
+                    float *dW3, float *db3, void *stream);
+/* Adam step on FP32 master weights p (bias-corrected, step t >= 1); refreshes the BF16
+ * copy p_bf16 (round to nearest even) when given. */
+int kf_mlp_adam(float *p, const float *g, float *m, float *v, int n, float lr, float beta1,
+                float beta2, float eps, int t, uint16_t *p_bf16, void *stream);
+int kf_cast_bf16(const float *p, int n, uint16_t *out, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* kernelmath: RFP Cholesky                                                  */

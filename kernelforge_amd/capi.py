@@ -56,6 +56,34 @@ SIGNATURES = {
     "kf_krr_solve_mixed": [_vp, _vp, _c_int, _c_double, _vp, _vp, _c_double, _c_int,
                            ctypes.POINTER(_c_int), ctypes.POINTER(_c_double),
                            ctypes.POINTER(_c_double), _vp],
+    "kf_local_kernel_jacobian": [_vp] * 7 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_kernel_jacobian_t": [_vp] * 7 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_kernel_hessian": [_vp] * 8 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_kernel_full": [_vp] * 8 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_kernel_hessian_symm": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
+    "kf_local_kernel_hessian_symm_rfp": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
+    "kf_local_kernel_full_symm": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
+    "kf_local_kernel_full_symm_rfp": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
+    "kf_local_alpha_desc": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
+    "kf_local_full_matvec": [_vp] * 9 + [_c_int] * 5 + [_c_double, _vp, _vp, _vp],
+    "kf_local_hessian_matvec": [_vp] * 8 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_jacobian_t_matvec": [_vp] * 7 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_local_full_diag_blocks": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
+    "kf_block_factor": [_vp, _c_int, _c_int, _vp, _c_int, _c_double, _vp, _vp, _vp, _vp],
+    "kf_block_apply": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp],
+    "kf_ef_pcg_rfp": [_vp, _c_int, _c_double, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp,
+                      _c_double, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp,
+                      _vp],
+    "kf_ef_pcg_matfree": [_vp] * 4 + [_c_int] * 3 + [_c_double, _c_double, _vp, _c_int, _vp, _vp,
+                                                      _c_double, _c_int, ctypes.POINTER(_c_int),
+                                                      ctypes.POINTER(_c_double), _vp, _vp],
+    "kf_pcg_unrolled_grad_rfp": [_vp, _c_int, _c_double, _vp, _c_int, _c_int, _vp, _c_int, _vp,
+                                 _c_int, ctypes.POINTER(_c_double), _vp, _vp],
+    "kf_mlp_forward": [_vp, _c_int, _c_int, _c_int, _c_int] + [_vp] * 10,
+    "kf_mlp_backward": [_vp, _c_int, _c_int, _c_int, _c_int] + [_vp] * 13,
+    "kf_mlp_adam": [_vp, _vp, _vp, _vp, _c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                    ctypes.c_float, _c_int, _vp, _vp],
+    "kf_cast_bf16": [_vp, _c_int, _vp, _vp],
     "kf_trttf": [_c_char, _c_char, _c_int, _vp, _c_int, _vp, _vp],
     "kf_tfttr": [_c_char, _c_char, _c_int, _vp, _vp, _c_int, _c_int, _vp],
 }
@@ -188,6 +216,24 @@ def local_kernel_symm_rfp(precision, X, Q, N, nm, max_atoms, rep, sigma, arf, st
 def local_kernel(precision, X1, X2, Q1, Q2, N1, N2, nm1, nm2, ma1, ma2, rep, sigma, K, stream=0):
     return check(lib().kf_local_kernel(precision, X1, X2, Q1, Q2, N1, N2, nm1, nm2, ma1, ma2, rep,
                                        sigma, K, stream or None), "kf_local_kernel")
+
+
+def local_kernel_full_symm_rfp(X, dX, Q, N, nm, max_atoms, rep, sigma, arf, stream=0):
+    return check(lib().kf_local_kernel_full_symm_rfp(X, dX, Q, N, nm, max_atoms, rep, sigma, arf,
+                                                     stream or None),
+                 "kf_local_kernel_full_symm_rfp")
+
+
+def local_alpha_desc(dX, Q, N, nm, max_atoms, rep, alpha, out, stream=0):
+    return check(lib().kf_local_alpha_desc(dX, Q, N, nm, max_atoms, rep, alpha, out,
+                                           stream or None), "kf_local_alpha_desc")
+
+
+def local_full_matvec(X1, dX1, X2, ad, aE, Q1, Q2, N1, N2, nm1, nm2, ma1, ma2, rep, sigma, E, F,
+                      stream=0):
+    return check(lib().kf_local_full_matvec(X1, dX1, X2, ad, aE, Q1, Q2, N1, N2, nm1, nm2, ma1,
+                                            ma2, rep, sigma, E, F, stream or None),
+                 "kf_local_full_matvec")
 
 
 def rfp_potrf(precision, arf, n, l2, stream=0):

@@ -52,12 +52,21 @@ enum WsSlot {
     WS_AUX1,
     WS_AUX2,
     WS_AUX3,
+    WS_SOLVE0,     // iterative-solver vectors (kept apart from the operator's slots)
+    WS_SOLVE1,     // solver reductions
+    WS_SOLVE2,     // solver metadata
+    WS_NET,        // preconditioner-network activations
     WS_NSLOTS
 };
 void *workspace(int slot, size_t bytes, hipStream_t stream);
 
 // Pinned host staging for small metadata round trips.
 void *host_staging(int slot, size_t bytes);
+
+// Q (nm x max_atoms) and N (nm) device arrays -> host vectors, through staging slot
+// `slot`; synchronises `stream`.
+int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &hQ,
+                  std::vector<int> &hN, hipStream_t stream, int slot);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -1,0 +1,771 @@
+// local_deriv.hip -- energy+force forms of the element-matched local Gaussian
+// kernel on gfx950: Jacobian (dK/dR2), transposed Jacobian (-dK/dR1), Hessian and
+// the full [[EE, EF], [FE, FF]] kernel, dense or symmetric RFP (TRANSR='N', UPLO='U').
+//
+// For a molecule pair (a, b) and every label-matched atom pair m = (i, j):
+//   d_m = x_ai - x_bj,  e_m = exp(-|d_m|^2 / 2 s^2),  c_m = e_m / s^2,  w_m = e_m / s^4
+//   EE = sum_m e_m
+//   FE = -sum_m c_m J_ai^T d_m                       (dK/dR_a, 3 n_a)
+//   EF = +sum_m c_m J_bj^T d_m                       (dK/dR_b, 3 n_b)
+//   FF = sum_i J_ai^T T_i - sum_m w_m v_m u_m^T      (3 n_a x 3 n_b)
+//        T_i = sum_{j matched to i} c_m J_bj,  v_m = J_ai^T d_m,  u_m = J_bj^T d_m
+// which is the reference's per-pair DGEMV/DGER + S_sum DGEMM scheme
+// (src/local_kernels.cpp:1171-1419 derivation, 2511-2815 full RFP) regrouped so
+// that the dominant term is one long contraction over K = (feature, atom).
+//
+// Two kernels per batch of molecule pairs, one 256-thread work-group per pair:
+//  * deriv_prep_kernel (FP64 VALU): the matched list, e_m, and the projections of
+//    J onto difference vectors streamed over the features (v_m, u_m for the
+//    Hessian forms; s_i = sum_j c_m d_m and t_j = sum_i c_m d_m for the E-only
+//    Jacobian forms).  Writes EE / EF / FE, and e_m, v_m, u_m to a per-pair slot.
+//  * deriv_ff_kernel (FP64 MFMA 16x16x4): sum_i J_ai^T T_i with (feature, atom)
+//    chunks of J_a and J_b staged in LDS and T built on the fly as the B operand,
+//    then the rank-1 sum as a second MFMA pass over the slot; writes FF.
+// Every output entry belongs to exactly one molecule pair, so no atomics are used
+// and results are deterministic.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "kf_common.hpp"
+#include "local_plan.hpp"
+
+namespace kf {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int NT = 256;            // threads per work-group (4 waves)
+constexpr int MAXA = 32;           // atoms per molecule supported here (3*MAXA <= 96 columns)
+constexpr int MAXM = MAXA * MAXA;  // matched atom pairs per molecule pair
+constexpr int VCH = 8;             // difference vectors per projection pass
+constexpr int FCH = 128;           // features per staged projection chunk
+constexpr int MAXPB = 6;           // 16-column blocks per side (96 / 16)
+constexpr size_t FF_LDS_BYTES = 64 * 1024;  // J chunks; two work-groups fit one CU
+
+enum DerivMode {
+    M_JAC = 0,         // (nm1, naq2)        EF
+    M_JAC_T,           // (naq1, nm2)        -FE
+    M_HESS,            // (naq1, naq2)       FF
+    M_FULL,            // (nm1+naq1, nm2+naq2)
+    M_HESS_SYMM,       // (naq, naq) dense, pairs a <= b mirrored
+    M_HESS_SYMM_RFP,   // RFP of order naq
+    M_FULL_SYMM,       // (nm+naq)^2 dense
+    M_FULL_SYMM_RFP,   // RFP of order nm+naq
+    M_DIAG_BLOCKS,     // per molecule b: the (E_b, F_b) diagonal block, bs x bs, bs = 1 + 3 ma
+};
+__host__ __device__ constexpr bool has_ff(int m) { return m >= M_HESS; }
+__host__ __device__ constexpr bool has_ee(int m) {
+    return m == M_FULL || m == M_FULL_SYMM || m == M_FULL_SYMM_RFP || m == M_DIAG_BLOCKS;
+}
+__host__ __device__ constexpr bool has_ef(int m) { return m == M_JAC || has_ee(m); }
+__host__ __device__ constexpr bool has_fe(int m) { return m == M_JAC_T || has_ee(m); }
+__host__ __device__ constexpr bool is_symm(int m) { return m >= M_HESS_SYMM && m != M_DIAG_BLOCKS; }
+__host__ __device__ constexpr bool is_rfp(int m) {
+    return m == M_HESS_SYMM_RFP || m == M_FULL_SYMM_RFP;
+}
+
+struct DerivArgs {
+    const double *x1, *dx1, *x2, *dx2;
+    const int *q1, *q2, *n1, *n2;
+    const int *offs1, *offs2;  // 3 * prefix sums of clamped atom counts
+    const int2 *pairs;         // (a, b) per work-group
+    int ma1, ma2, rep;
+    double cexp, inv_s2, inv_s4;
+    double *out;
+    size_t ld;     // dense row stride, or the RFP order
+    int ebase1;    // energy rows preceding the force rows (nm1 for the full forms)
+    int ebase2;    // energy columns preceding the force columns
+    double *slot;  // per-pair scratch: e[maxm] | V[maxm][ldv] | U[maxm][ldu]
+    size_t slot_len;
+    int maxm, ldv, ldu;
+};
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Work-group -> pair index.  Consecutive pairs share their first molecule; the
+// hardware deals work-groups round-robin over the 8 XCDs, so this keeps runs of
+// consecutive pairs (and their J_a) on one XCD's L2.
+__device__ inline int pair_slot() {
+    const int g = blockIdx.x, G = gridDim.x;
+    return (G % 8 == 0) ? (g % 8) * (G / 8) + g / 8 : g;
+}
+
+// ---------------------------------------------------------------------------
+// Output addressing per form.  (a, b) is the pair, oa/ob the force offsets.
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct Out {
+    double *o;
+    size_t ld;
+    int a, b, oa, ob, e1, e2;
+    __device__ void put(size_t r, size_t c, double v) const {
+        if (MODE == M_DIAG_BLOCKS)
+            o[((size_t)a * ld + r) * ld + c] = v;  // block a, local (r, c)
+        else if (is_rfp(MODE))
+            o[rfp_upper(ld, r, c)] = v;  // caller guarantees r <= c
+        else
+            o[r * ld + c] = v;
+    }
+    __device__ void ee(double v) const {
+        if (MODE == M_DIAG_BLOCKS) put(0, 0, v);
+        if (MODE == M_FULL) put(a, b, v);
+        if (MODE == M_FULL_SYMM) { put(a, b, v); put(b, a, v); }
+        if (MODE == M_FULL_SYMM_RFP) put(a, b, v);
+    }
+    // EF[r] = dK/dR_b; row a, force column ob + r.
+    __device__ void ef(int r, double v) const {
+        const size_t col = (size_t)e2 + ob + r;
+        if (MODE == M_DIAG_BLOCKS) put(0, 1 + r, v);
+        if (MODE == M_JAC || MODE == M_FULL || MODE == M_FULL_SYMM_RFP) put(a, col, v);
+        if (MODE == M_FULL_SYMM) {
+            put(a, col, v);
+            if (a != b) put(col, a, v);
+        }
+    }
+    // FE[r] = dK/dR_a; force row oa + r, column b.
+    __device__ void fe(int r, double v) const {
+        const size_t row = (size_t)e1 + oa + r;
+        if (MODE == M_DIAG_BLOCKS) put(1 + r, 0, v);
+        if (MODE == M_JAC_T) put(oa + r, b, -v);
+        if (MODE == M_FULL) put(row, b, v);
+        if (MODE == M_FULL_SYMM) {
+            put(row, b, v);
+            if (a != b) put(b, row, v);
+        }
+        if (MODE == M_FULL_SYMM_RFP && a != b) put(b, row, v);
+    }
+    __device__ void ff(int r, int t, double v) const {
+        const size_t row = (size_t)e1 + oa + r, col = (size_t)e2 + ob + t;
+        if (MODE == M_DIAG_BLOCKS) put(1 + r, 1 + t, v);
+        if (MODE == M_HESS || MODE == M_FULL) put(row, col, v);
+        if (MODE == M_HESS_SYMM || MODE == M_FULL_SYMM) {
+            put(row, col, v);
+            if (a != b) put(col, row, v);
+        }
+        if (MODE == M_HESS_SYMM_RFP || MODE == M_FULL_SYMM_RFP) {
+            if (a != b || r <= t) put(row, col, v);
+        }
+    }
+};
+
+// Matched list of one pair, built by thread 0 in i-major order (m -> (i, j)),
+// with per-i ranges and a j-major permutation.
+struct PairLists {
+    int ml[MAXM];        // (i << 8) | j
+    int ist[MAXA + 1];   // i-major ranges
+    int jst[MAXA + 1];   // j-major ranges into jl
+    int jl[MAXM];        // m indices grouped by j (ascending i inside a group)
+    int qa[MAXA], qb[MAXA];
+    int cj[MAXA];        // per-j fill cursors
+    int np;
+};
+
+__device__ void build_lists(PairLists &L, const int *q1, const int *q2, int a, int b, int ma1,
+                            int ma2, int na, int nb) {
+    const int tid = threadIdx.x;
+    if (tid < na) L.qa[tid] = q1[(size_t)a * ma1 + tid];
+    if (tid < nb) L.qb[tid] = q2[(size_t)b * ma2 + tid];
+    __syncthreads();
+    if (tid == 0) {
+        int cnt = 0;
+        int *cj = L.cj;
+        for (int j = 0; j < nb; ++j) cj[j] = 0;
+        for (int i = 0; i < na; ++i) {
+            L.ist[i] = cnt;
+            const int qi = L.qa[i];
+            for (int j = 0; j < nb; ++j)
+                if (L.qb[j] == qi) { L.ml[cnt++] = (i << 8) | j; cj[j]++; }
+        }
+        L.ist[na] = cnt;
+        int acc = 0;
+        for (int j = 0; j < nb; ++j) { L.jst[j] = acc; acc += cj[j]; cj[j] = L.jst[j]; }
+        L.jst[nb] = acc;
+        for (int m = 0; m < cnt; ++m) L.jl[cj[L.ml[m] & 255]++] = m;
+        L.np = cnt;
+    }
+    __syncthreads();
+}
+
+// acc[v] (for this thread's column p and feature slice) += sum_f J[f, p] vec_v[f],
+// vectors produced chunk-wise by mk(v, f); emit(v, p, pb, value) after the
+// cross-slice reduction.  J rows have stride ldj; ncol columns are live.
+template <int NV, class MakeVec, class Emit>
+__device__ void project(const double *J, int ldj, int ncol, int rep, int nv, MakeVec mk,
+                        Emit emit, double (*dv)[FCH], double (*red)[VCH][64]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int pb = 0; pb * 64 < ncol; ++pb) {
+        const int p = pb * 64 + lane;
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        for (int f0 = 0; f0 < rep; f0 += FCH) {
+            const int fn = min(FCH, rep - f0);
+            __syncthreads();
+            for (int e = tid; e < nv * FCH; e += NT) {
+                const int v = e / FCH, f = e - v * FCH;
+                dv[v][f] = f < fn ? mk(v, f0 + f) : 0.0;
+            }
+            __syncthreads();
+            if (p < ncol) {
+                const double *jp = J + (size_t)f0 * ldj + p;
+                for (int f = wv; f < fn; f += 4) {
+                    const double jv = jp[(size_t)f * ldj];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] = fma(jv, dv[v][f], acc[v]);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) red[wv][v][lane] = acc[v];
+        __syncthreads();
+        for (int v = wv; v < nv; v += 4)
+            if (p < ncol)
+                emit(v, p, pb, red[0][v][lane] + red[1][v][lane] + red[2][v][lane] +
+                                   red[3][v][lane]);
+        __syncthreads();
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) {
+    __shared__ PairLists L;
+    __shared__ double em[MAXM];
+    __shared__ double dv[VCH][FCH];
+    __shared__ double red[4][VCH][64];
+    __shared__ double part[4][2 * 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int t = pair0 + pair_slot();
+    const int a = A.pairs[t].x, b = A.pairs[t].y;
+    const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
+    const int rep = A.rep, nca = 3 * na, ncb = 3 * nb;
+    build_lists(L, A.q1, A.q2, a, b, A.ma1, A.ma2, na, nb);
+    const int np = L.np;
+    const double *xa0 = A.x1 + (size_t)a * A.ma1 * rep;
+    const double *xb0 = A.x2 + (size_t)b * A.ma2 * rep;
+
+    // e_m from the exact squared distance (one wave per matched pair).
+    for (int m = wv; m < np; m += 4) {
+        const double *xa = xa0 + (size_t)(L.ml[m] >> 8) * rep;
+        const double *xb = xb0 + (size_t)(L.ml[m] & 255) * rep;
+        double s = 0.0;
+        for (int f = lane; f < rep; f += 64) {
+            const double d = xa[f] - xb[f];
+            s = fma(d, d, s);
+        }
+        s = wave_sum(s);
+        if (lane == 0) em[m] = exp(s * A.cexp);
+    }
+    __syncthreads();
+
+    Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+    double *slot = has_ff(MODE) ? A.slot + (size_t)(t - pair0) * A.slot_len : nullptr;
+    if (has_ff(MODE))
+        for (int m = tid; m < np; m += NT) slot[m] = em[m];
+    if (has_ee(MODE) && tid == 0) {
+        double s = 0.0;
+        for (int m = 0; m < np; ++m) s += em[m];
+        out.ee(s);
+    }
+    const double inv_s2 = A.inv_s2;
+    // Per-thread partial sums of FE (pb 0/1 for column pb*64+lane).
+    double fe_acc[2] = {0.0, 0.0};
+    double ef_acc[2] = {0.0, 0.0};
+
+    // ---------------- A side: J_ai projections ----------------
+    if (has_fe(MODE) || has_ff(MODE)) {
+        const size_t lda = 3 * (size_t)A.ma1;
+        for (int i = 0; i < na; ++i) {
+            const int m0 = L.ist[i], m1 = L.ist[i + 1];
+            if (m1 == m0) continue;
+            const double *J = A.dx1 + ((size_t)a * A.ma1 + i) * rep * lda;
+            const double *xa = xa0 + (size_t)i * rep;
+            if (has_ff(MODE)) {
+                double *V = slot + A.maxm;
+                for (int c0 = m0; c0 < m1; c0 += VCH) {
+                    const int nv = min(VCH, m1 - c0);
+                    project<VCH>(
+                        J, (int)lda, nca, rep, nv,
+                        [&](int v, int f) {
+                            return xa[f] - xb0[(size_t)(L.ml[c0 + v] & 255) * rep + f];
+                        },
+                        [&](int v, int p, int pb, double val) {
+                            V[(size_t)(c0 + v) * A.ldv + p] = val;
+                            const double c = em[c0 + v] * inv_s2 * val;
+                            if (pb == 0) fe_acc[0] -= c; else fe_acc[1] -= c;
+                        },
+                        dv, red);
+                }
+            } else {
+                project<1>(
+                    J, (int)lda, nca, rep, 1,
+                    [&](int, int f) {
+                        double s = 0.0;
+                        for (int m = m0; m < m1; ++m)
+                            s += em[m] * (xa[f] - xb0[(size_t)(L.ml[m] & 255) * rep + f]);
+                        return s * inv_s2;
+                    },
+                    [&](int, int, int pb, double val) {
+                        if (pb == 0) fe_acc[0] -= val; else fe_acc[1] -= val;
+                    },
+                    dv, red);
+            }
+        }
+    }
+    // ---------------- B side: J_bj projections ----------------
+    if (has_ff(MODE) || has_ef(MODE)) {
+        const size_t ldb = 3 * (size_t)A.ma2;
+        for (int j = 0; j < nb; ++j) {
+            const int g0 = L.jst[j], g1 = L.jst[j + 1];
+            if (g1 == g0) continue;
+            const double *J = A.dx2 + ((size_t)b * A.ma2 + j) * rep * ldb;
+            const double *xb = xb0 + (size_t)j * rep;
+            if (has_ff(MODE)) {
+                double *U = slot + A.maxm + (size_t)A.maxm * A.ldv;
+                for (int c0 = g0; c0 < g1; c0 += VCH) {
+                    const int nv = min(VCH, g1 - c0);
+                    project<VCH>(
+                        J, (int)ldb, ncb, rep, nv,
+                        [&](int v, int f) {
+                            return xa0[(size_t)(L.ml[L.jl[c0 + v]] >> 8) * rep + f] - xb[f];
+                        },
+                        [&](int v, int p, int pb, double val) {
+                            const int m = L.jl[c0 + v];
+                            U[(size_t)m * A.ldu + p] = val;
+                            const double c = em[m] * inv_s2 * val;
+                            if (pb == 0) ef_acc[0] += c; else ef_acc[1] += c;
+                        },
+                        dv, red);
+                }
+            } else {
+                project<1>(
+                    J, (int)ldb, ncb, rep, 1,
+                    [&](int, int f) {
+                        double s = 0.0;
+                        for (int g = g0; g < g1; ++g) {
+                            const int m = L.jl[g];
+                            s += em[m] * (xa0[(size_t)(L.ml[m] >> 8) * rep + f] - xb[f]);
+                        }
+                        return s * inv_s2;
+                    },
+                    [&](int, int, int pb, double val) {
+                        if (pb == 0) ef_acc[0] += val; else ef_acc[1] += val;
+                    },
+                    dv, red);
+            }
+        }
+    }
+    // ---------------- reduce the partial FE / EF and write ----------------
+    if (has_fe(MODE)) {
+        part[wv][lane] = fe_acc[0];
+        part[wv][64 + lane] = fe_acc[1];
+        __syncthreads();
+        for (int r = tid; r < nca; r += NT)
+            out.fe(r, part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+        __syncthreads();
+    }
+    if (has_ef(MODE)) {
+        part[wv][lane] = ef_acc[0];
+        part[wv][64 + lane] = ef_acc[1];
+        __syncthreads();
+        for (int r = tid; r < ncb; r += NT)
+            out.ef(r, part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FF = sum_(f,i) J_a[i,f,:]^T T[(f,i),:]  -  sum_m w_m v_m u_m^T   on FP64 MFMA.
+// LDS: JA[fc][NI][NPp] | JB[fc][nb][NQp] (dynamic), zero padded.
+// Wave w owns q-blocks {qb : qb == w mod 4} (all p-blocks) when there are at least
+// 3 q-blocks; with 1 or 2 q-blocks the waves also split the p-blocks.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, int fc) {
+    extern __shared__ double lds[];
+    __shared__ int qa[MAXA], qb_[MAXA], ist[MAXA + 1], mj[MAXM];
+    __shared__ double mc[MAXM];
+    __shared__ int s_np;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, col = lane & 15,
+              kr = lane >> 4;
+    const int t = pair0 + pair_slot();
+    const int a = A.pairs[t].x, b = A.pairs[t].y;
+    const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
+    const int rep = A.rep, nca = 3 * na, ncb = 3 * nb;
+    if (na == 0 || nb == 0) return;
+    const double *slot = A.slot + (size_t)(t - pair0) * A.slot_len;
+    if (tid < na) qa[tid] = A.q1[(size_t)a * A.ma1 + tid];
+    if (tid < nb) qb_[tid] = A.q2[(size_t)b * A.ma2 + tid];
+    __syncthreads();
+    if (tid == 0) {
+        int cnt = 0;
+        for (int i = 0; i < na; ++i) {
+            ist[i] = cnt;
+            for (int j = 0; j < nb; ++j)
+                if (qb_[j] == qa[i]) mj[cnt++] = j;
+        }
+        ist[na] = cnt;
+        s_np = cnt;
+    }
+    __syncthreads();
+    const int np = s_np;
+    for (int m = tid; m < np; m += NT) mc[m] = slot[m] * A.inv_s2;
+
+    const int NI = (na + 3) & ~3, npb = (nca + 15) >> 4, nqb = (ncb + 15) >> 4;
+    const int NPp = npb * 16, NQp = nqb * 16;
+    // wave -> (q-blocks, p-block subset)
+    int qlist[2] = {-1, -1}, p0 = 0, pstep = 1;
+    if (nqb >= 3) {
+        qlist[0] = wv < nqb ? wv : -1;
+        qlist[1] = wv + 4 < nqb ? wv + 4 : -1;
+    } else {
+        const int split = 4 / nqb;  // 4 (nqb = 1) or 2 (nqb = 2)
+        qlist[0] = wv % nqb;
+        p0 = wv / nqb;
+        pstep = split;
+    }
+    v4d acc[MAXPB][2];
+#pragma unroll
+    for (int x = 0; x < MAXPB; ++x) acc[x][0] = acc[x][1] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    double *JA = lds, *JB = lds + (size_t)fc * NI * NPp;
+    const size_t lda = 3 * (size_t)A.ma1, ldb = 3 * (size_t)A.ma2;
+    const double *dxa = A.dx1 + (size_t)a * A.ma1 * rep * lda;
+    const double *dxb = A.dx2 + (size_t)b * A.ma2 * rep * ldb;
+
+    for (int f0 = 0; f0 < rep; f0 += fc) {
+        const int fn = min(fc, rep - f0);
+        __syncthreads();
+        // rows (f, i) of NPp columns: lanes over columns, waves over rows
+        for (int row = wv; row < fc * NI; row += 4) {
+            const int f = row / NI, i = row - f * NI;
+            const bool live = f < fn && i < na;
+            const double *src = dxa + ((size_t)i * rep + f0 + f) * lda;
+            for (int p = lane; p < NPp; p += 64)
+                JA[(size_t)row * NPp + p] = (live && p < nca) ? src[p] : 0.0;
+        }
+        for (int row = wv; row < fc * nb; row += 4) {
+            const int f = row / nb, j = row - f * nb;
+            const bool live = f < fn;
+            const double *src = dxb + ((size_t)j * rep + f0 + f) * ldb;
+            for (int q = lane; q < NQp; q += 64)
+                JB[(size_t)row * NQp + q] = (live && q < ncb) ? src[q] : 0.0;
+        }
+        __syncthreads();
+        const int ksteps = fn * NI / 4;
+        for (int s = 0; s < ksteps; ++s) {
+            const int kk = 4 * s, f = kk / NI, i = kk - f * NI + kr;
+            const double *arow = JA + ((size_t)f * NI + i) * NPp + col;
+            const double *brow = JB + (size_t)f * nb * NQp + col;
+            const int mb = i < na ? ist[i] : 0, me = i < na ? ist[i + 1] : 0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int qb = qlist[k];
+                if (qb < 0) continue;
+                double bv = 0.0;
+                for (int m = mb; m < me; ++m) bv = fma(mc[m], brow[mj[m] * NQp + qb * 16], bv);
+#pragma unroll
+                for (int x = 0; x < MAXPB; ++x) {
+                    const int pb = p0 + x * pstep;
+                    if (pb < npb)
+                        acc[x][k] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[pb * 16], bv,
+                                                                         acc[x][k], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // rank-1 part: acc -= sum_m (w_m v_m) u_m^T, K = matched pairs
+    {
+        const double *V = slot + A.maxm, *U = V + (size_t)A.maxm * A.ldv;
+        for (int m0 = 0; m0 < np; m0 += 4) {
+            const int m = m0 + kr;
+            const bool ok = m < np;
+            const double w = ok ? -mc[m] * A.inv_s2 : 0.0;  // -e/s^4
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int qb = qlist[k];
+                if (qb < 0) continue;
+                const int q = qb * 16 + col;
+                const double bv = (ok && q < ncb) ? U[(size_t)m * A.ldu + q] : 0.0;
+#pragma unroll
+                for (int x = 0; x < MAXPB; ++x) {
+                    const int pb = p0 + x * pstep;
+                    if (pb < npb) {
+                        const int p = pb * 16 + col;
+                        const double av = (ok && p < nca) ? w * V[(size_t)m * A.ldv + p] : 0.0;
+                        acc[x][k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[x][k], 0,
+                                                                         0, 0);
+                    }
+                }
+            }
+        }
+    }
+    Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int qb = qlist[k];
+        if (qb < 0) continue;
+        const int tc = qb * 16 + col;
+#pragma unroll
+        for (int x = 0; x < MAXPB; ++x) {
+            const int pb = p0 + x * pstep;
+            if (pb >= npb) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = pb * 16 + mfma16_row_f64(lane, v);
+                if (r < nca && tc < ncb) out.ff(r, tc, acc[x][k][v]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct SideMeta {
+    std::vector<int> q, n, offs;  // offs: nm + 1
+    std::vector<int> hist;        // per molecule label histogram over `labels`
+    int naq = 0;
+};
+
+void side_meta(const std::vector<int> &hQ, const std::vector<int> &hN, int nm, int ma,
+               const std::vector<int> &labels, SideMeta &s) {
+    s.q = hQ;
+    s.n = hN;
+    s.offs.assign(nm + 1, 0);
+    const int L = (int)labels.size();
+    s.hist.assign((size_t)nm * L, 0);
+    int acc = 0;
+    for (int m = 0; m < nm; ++m) {
+        const int na = std::min(std::max(hN[m], 0), ma);
+        s.offs[m] = acc;
+        acc += 3 * na;
+        for (int i = 0; i < na; ++i) {
+            const int l = int(std::lower_bound(labels.begin(), labels.end(), hQ[(size_t)m * ma + i]) -
+                              labels.begin());
+            if (l < L && labels[l] == hQ[(size_t)m * ma + i]) s.hist[(size_t)m * L + l]++;
+        }
+    }
+    s.offs[nm] = acc;
+    s.naq = acc;
+}
+
+template <int MODE>
+int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1,
+                 const SideMeta &s2, int nlab, hipStream_t stream) {
+    const size_t npairs = pairs.size();
+    if (npairs == 0) return KF_OK;
+    // offsets + pair list to the device (one staging copy)
+    const size_t n_off = s1.offs.size() + s2.offs.size();
+    const size_t pair_off = (sizeof(int) * n_off + 15) / 16 * 16;
+    const size_t bytes = pair_off + sizeof(int2) * npairs;
+    char *dev = static_cast<char *>(workspace(WS_PLAN, bytes, stream));
+    char *stage = static_cast<char *>(host_staging(WS_PLAN, bytes));
+    KF_REQUIRE(dev && stage, KF_ERR_NOMEM, "workspace allocation failed (%zu B)", bytes);
+    std::copy(s1.offs.begin(), s1.offs.end(), reinterpret_cast<int *>(stage));
+    std::copy(s2.offs.begin(), s2.offs.end(), reinterpret_cast<int *>(stage) + s1.offs.size());
+    std::copy(pairs.begin(), pairs.end(), reinterpret_cast<int2 *>(stage + pair_off));
+    KF_HIP_TRY(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, stream));
+    A.offs1 = reinterpret_cast<int *>(dev);
+    A.offs2 = A.offs1 + s1.offs.size();
+    A.pairs = reinterpret_cast<const int2 *>(dev + pair_off);
+
+    size_t batch = npairs;
+    int fc = 1;
+    if (has_ff(MODE)) {
+        // slot layout and the largest matched count over the pairs
+        int maxm = 1;
+        for (const int2 &p : pairs) {
+            int c = 0;
+            for (int l = 0; l < nlab; ++l)
+                c += s1.hist[(size_t)p.x * nlab + l] * s2.hist[(size_t)p.y * nlab + l];
+            maxm = std::max(maxm, c);
+        }
+        A.maxm = maxm;
+        A.ldv = (3 * A.ma1 + 15) / 16 * 16;
+        A.ldu = (3 * A.ma2 + 15) / 16 * 16;
+        A.slot_len = (size_t)maxm * (1 + A.ldv + A.ldu);
+        const size_t budget = size_t(1) << 30;
+        batch = std::max<size_t>(1, std::min(npairs, budget / (A.slot_len * sizeof(double))));
+        A.slot = static_cast<double *>(
+            workspace(WS_AUX2, batch * A.slot_len * sizeof(double), stream));
+        KF_REQUIRE(A.slot, KF_ERR_NOMEM, "scratch allocation failed");
+        const int NI = (A.ma1 + 3) & ~3;
+        const int per_f = NI * ((3 * A.ma1 + 15) / 16 * 16) + A.ma2 * ((3 * A.ma2 + 15) / 16 * 16);
+        fc = std::max(1, std::min(8, int(FF_LDS_BYTES / sizeof(double) / per_f)));
+        fc = std::min(fc, A.rep);
+        // keep fc * NI a multiple of 4 k-steps: NI already is
+    } else {
+        batch = std::min<size_t>(npairs, size_t(1) << 20);
+    }
+    const int NI = (A.ma1 + 3) & ~3;
+    const size_t lds_ff = sizeof(double) * (size_t)fc *
+                          (NI * ((3 * A.ma1 + 15) / 16 * 16) + A.ma2 * ((3 * A.ma2 + 15) / 16 * 16));
+    for (size_t p0 = 0; p0 < npairs; p0 += batch) {
+        const unsigned nb = (unsigned)std::min(batch, npairs - p0);
+        hipLaunchKernelGGL(deriv_prep_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0);
+        KF_HIP_TRY(hipGetLastError());
+        if (has_ff(MODE)) {
+            hipLaunchKernelGGL(deriv_ff_kernel<MODE>, dim3(nb), dim3(NT), lds_ff, stream, A,
+                               (int)p0, fc);
+            KF_HIP_TRY(hipGetLastError());
+        }
+    }
+    return KF_OK;
+}
+
+template <int MODE>
+int deriv_impl(const double *X1, const double *dX1, const double *X2, const double *dX2,
+               const int *Q1, const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
+               int ma1, int ma2, int rep, double sigma, double *out, hipStream_t stream) {
+    KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
+    KF_REQUIRE(nm1 >= 0 && nm2 >= 0 && ma1 >= 0 && ma2 >= 0 && rep >= 1, KF_ERR_ARG,
+               "bad dimensions");
+    KF_REQUIRE(ma1 <= MAXA && ma2 <= MAXA, KF_ERR_BASIS,
+               "derivative kernels support at most %d atoms per molecule (got %d, %d)", MAXA,
+               ma1, ma2);
+    if (nm1 == 0 || nm2 == 0) return KF_OK;
+    KF_REQUIRE(X1 && X2 && Q1 && Q2 && N1 && N2 && out, KF_ERR_ARG, "null pointer");
+    const bool need1 = has_fe(MODE) || has_ff(MODE), need2 = has_ef(MODE) || has_ff(MODE);
+    KF_REQUIRE((!need1 || dX1) && (!need2 || dX2), KF_ERR_ARG, "null Jacobian pointer");
+    std::vector<int> q1, n1, q2, n2;
+    int rc;
+    if ((rc = read_metadata(Q1, N1, nm1, ma1, q1, n1, stream, WS_AUX0))) return rc;
+    if (is_symm(MODE) || MODE == M_DIAG_BLOCKS) {
+        q2 = q1;
+        n2 = n1;
+    } else if ((rc = read_metadata(Q2, N2, nm2, ma2, q2, n2, stream, WS_AUX1))) {
+        return rc;
+    }
+    std::vector<int> labels;
+    {
+        std::vector<int> l1 = distinct_labels(q1.data(), n1.data(), nm1, ma1),
+                         l2 = distinct_labels(q2.data(), n2.data(), nm2, ma2);
+        labels.resize(l1.size() + l2.size());
+        labels.erase(std::set_union(l1.begin(), l1.end(), l2.begin(), l2.end(), labels.begin()),
+                     labels.end());
+    }
+    SideMeta s1, s2;
+    side_meta(q1, n1, nm1, ma1, labels, s1);
+    side_meta(q2, n2, nm2, ma2, labels, s2);
+
+    DerivArgs A{};
+    A.x1 = X1; A.dx1 = dX1; A.x2 = X2; A.dx2 = dX2;
+    A.q1 = Q1; A.q2 = Q2; A.n1 = N1; A.n2 = N2;
+    A.ma1 = ma1; A.ma2 = ma2; A.rep = rep;
+    A.cexp = -1.0 / (2.0 * sigma * sigma);
+    A.inv_s2 = 1.0 / (sigma * sigma);
+    A.inv_s4 = A.inv_s2 * A.inv_s2;
+    A.out = out;
+    switch (MODE) {
+        case M_JAC: A.ld = s2.naq; break;
+        case M_JAC_T: A.ld = nm2; break;
+        case M_HESS: A.ld = s2.naq; break;
+        case M_FULL: A.ld = (size_t)nm2 + s2.naq; A.ebase1 = nm1; A.ebase2 = nm2; break;
+        case M_HESS_SYMM:
+        case M_HESS_SYMM_RFP: A.ld = s1.naq; break;
+        case M_DIAG_BLOCKS: A.ld = 1 + 3 * (size_t)ma1; break;
+        default: A.ld = (size_t)nm1 + s1.naq; A.ebase1 = nm1; A.ebase2 = nm1; break;
+    }
+    std::vector<int2> pairs;
+    if (MODE == M_DIAG_BLOCKS) {
+        pairs.reserve(nm1);
+        for (int a = 0; a < nm1; ++a) pairs.push_back(make_int2(a, a));
+    } else if (is_symm(MODE)) {
+        pairs.reserve((size_t)nm1 * (nm1 + 1) / 2);
+        for (int a = 0; a < nm1; ++a)
+            for (int b = a; b < nm1; ++b) pairs.push_back(make_int2(a, b));
+    } else {
+        pairs.reserve((size_t)nm1 * nm2);
+        for (int a = 0; a < nm1; ++a)
+            for (int b = 0; b < nm2; ++b) pairs.push_back(make_int2(a, b));
+    }
+    // the Hessian forms of zero-size sets have nothing to write
+    if ((MODE == M_HESS || MODE == M_HESS_SYMM || MODE == M_HESS_SYMM_RFP) &&
+        (s1.naq == 0 || s2.naq == 0))
+        return KF_OK;
+    ScopedTimer timer(TM_FULL, stream);
+    return deriv_launch<MODE>(A, pairs, s1, s2, (int)labels.size(), stream);
+}
+
+}  // namespace
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" int kf_local_kernel_jacobian(const double *X1, const double *X2, const double *dX2,
+                                        const int *Q1, const int *Q2, const int *N1,
+                                        const int *N2, int nm1, int nm2, int max_atoms1,
+                                        int max_atoms2, int rep, double sigma, double *K,
+                                        void *stream) {
+    return deriv_impl<M_JAC>(X1, nullptr, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
+                             max_atoms2, rep, sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_jacobian_t(const double *X1, const double *X2, const double *dX1,
+                                          const int *Q1, const int *Q2, const int *N1,
+                                          const int *N2, int nm1, int nm2, int max_atoms1,
+                                          int max_atoms2, int rep, double sigma, double *K,
+                                          void *stream) {
+    return deriv_impl<M_JAC_T>(X1, dX1, X2, nullptr, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
+                               max_atoms2, rep, sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_hessian(const double *X1, const double *X2, const double *dX1,
+                                       const double *dX2, const int *Q1, const int *Q2,
+                                       const int *N1, const int *N2, int nm1, int nm2,
+                                       int max_atoms1, int max_atoms2, int rep, double sigma,
+                                       double *K, void *stream) {
+    return deriv_impl<M_HESS>(X1, dX1, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1, max_atoms2,
+                              rep, sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_full(const double *X1, const double *X2, const double *dX1,
+                                    const double *dX2, const int *Q1, const int *Q2,
+                                    const int *N1, const int *N2, int nm1, int nm2,
+                                    int max_atoms1, int max_atoms2, int rep, double sigma,
+                                    double *K, void *stream) {
+    return deriv_impl<M_FULL>(X1, dX1, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1, max_atoms2,
+                              rep, sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_hessian_symm(const double *X, const double *dX, const int *Q,
+                                            const int *N, int nm, int max_atoms, int rep,
+                                            double sigma, double *K, void *stream) {
+    return deriv_impl<M_HESS_SYMM>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
+                                   sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_hessian_symm_rfp(const double *X, const double *dX, const int *Q,
+                                                const int *N, int nm, int max_atoms, int rep,
+                                                double sigma, double *arf, void *stream) {
+    return deriv_impl<M_HESS_SYMM_RFP>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms,
+                                       rep, sigma, arf, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_full_symm(const double *X, const double *dX, const int *Q,
+                                         const int *N, int nm, int max_atoms, int rep,
+                                         double sigma, double *K, void *stream) {
+    return deriv_impl<M_FULL_SYMM>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
+                                   sigma, K, as_stream(stream));
+}
+
+extern "C" int kf_local_kernel_full_symm_rfp(const double *X, const double *dX, const int *Q,
+                                             const int *N, int nm, int max_atoms, int rep,
+                                             double sigma, double *arf, void *stream) {
+    return deriv_impl<M_FULL_SYMM_RFP>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms,
+                                       rep, sigma, arf, as_stream(stream));
+}
+
+extern "C" int kf_local_full_diag_blocks(const double *X, const double *dX, const int *Q,
+                                         const int *N, int nm, int max_atoms, int rep,
+                                         double sigma, double *blocks, void *stream) {
+    // entries beyond a molecule's 1 + 3 N[b] live rows/columns are not written
+    return deriv_impl<M_DIAG_BLOCKS>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
+                                     sigma, blocks, as_stream(stream));
+}

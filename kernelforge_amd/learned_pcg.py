@@ -1,0 +1,249 @@
+"""Learned block-Jacobi PCG for the energy+force kernel system, on the GPU.
+
+The system is (K_full + lambda I) alpha = [E; -F] in the reference's full-kernel order
+(python/kernelforge/models/krr.py:140-166 builds the same labels and solves it with
+cho_solve_rfp).  Here it can instead be solved by PCG driven by the matrix-free full
+matvec (kf_ef_pcg_matfree), preconditioned by per-conformer (E_b, F_b) blocks
+
+    P_b = (K_bb + lambda I + diag(s_b))^-1,   s_b = d_b * exp(H_MID + H_AMP tanh(y_b)),
+
+where d_b is |diag(K_bb)| and y_b = MLP(features_b) is per coordinate, so the relative
+shift stays within [7.6e-9, 0.1] (a bounded range keeps the unrolled loss smooth;
+relative shifts outside it either vanish below rounding or swamp the block), predicted by a 2-hidden-layer ReLU network (width 128, BF16 weights, FP32 accumulation)
+from the conformer's mean descriptor, log sigma and log10 lambda.  The network is
+trained on kernel systems built from random subsets of the training conformers, by
+Adam on the loss log(||r_10|| / ||y||) of 10 unrolled PCG steps, differentiated in
+reverse mode on the GPU (kf_pcg_unrolled_grad_rfp) and back-propagated through the
+network by the HIP kernels in csrc/mlp.hip.  Kernels, Jacobians, blocks, vectors and
+weights live on the device; only the small per-conformer arrays (features, the
+network's log-shifts and their gradients, nm x bs) pass through the host between calls.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import capi
+
+SIGMA_DEFAULT = 2.0
+H_MID, H_AMP = -10.5, 8.2
+INIT_REL_SHIFT = 1e-6
+
+
+class DeviceSystem:
+    """Descriptors, Jacobians, labels and diagonal blocks of one E+F training set."""
+
+    def __init__(self, coords, Z, energies, forces, sigma=SIGMA_DEFAULT, stream=0):
+        coords = np.ascontiguousarray(coords, np.float64)
+        nm, ma = coords.shape[0], coords.shape[1]
+        self.nm, self.ma, self.rep, self.sigma = nm, ma, capi.rep_size(), sigma
+        self.Q = np.ascontiguousarray(np.tile(Z, (nm, 1)), np.int32)
+        self.N = np.full(nm, ma, np.int32)
+        self.bs = 1 + 3 * ma
+        self.n = nm + 3 * ma * nm
+        DA = capi.DeviceArray
+        self.dC, self.dQ, self.dN = DA.from_numpy(coords), DA.from_numpy(self.Q), DA.from_numpy(self.N)
+        self.X = DA((nm, ma, self.rep))
+        self.dX = DA((nm, ma, self.rep, 3 * ma))
+        capi.fchl19(self.dC.ptr, self.dQ.ptr, self.dN.ptr, nm, ma, self.X.ptr, self.dX.ptr,
+                    stream)
+        self.y = DA.from_numpy(np.concatenate([energies, -np.asarray(forces).reshape(-1)]))
+        self.blocks = DA((nm, self.bs, self.bs))
+        self.blocks.zero(stream)
+        capi.check(capi.lib().kf_local_full_diag_blocks(self.X.ptr, self.dX.ptr, self.dQ.ptr,
+                                                        self.dN.ptr, nm, ma, self.rep, sigma,
+                                                        self.blocks.ptr, stream or None),
+                   "kf_local_full_diag_blocks")
+        capi.synchronize(stream)
+        b = self.blocks.numpy()
+        self.dscale = np.abs(np.einsum("bii->bi", b))  # |diag K_bb|, (nm, bs)
+        self.arf = None
+
+    def build_rfp(self, stream=0):
+        """Explicit E+F kernel in RFP (for training systems and for the direct solve)."""
+        if self.arf is None:
+            self.arf = capi.DeviceArray(self.n * (self.n + 1) // 2)
+        capi.local_kernel_full_symm_rfp(self.X.ptr, self.dX.ptr, self.dQ.ptr, self.dN.ptr,
+                                        self.nm, self.ma, self.rep, self.sigma, self.arf.ptr,
+                                        stream)
+        return self.arf
+
+    def free_rfp(self):
+        self.arf = None
+
+    def features(self, lam):
+        """Per-conformer network input (mean descriptor, log sigma, log10 lambda)."""
+        X = self.X.numpy()
+        mean = X.mean(axis=1)
+        extra = np.tile([math.log(self.sigma), math.log10(lam)], (self.nm, 1))
+        return np.concatenate([mean, extra], axis=1)
+
+
+class ShiftNet:
+    """The MLP with FP32 master weights, BF16 device copies and Adam state."""
+
+    NAMES = ("W1", "b1", "W2", "b2", "W3", "b3")
+
+    def __init__(self, d0, d3, hidden=128, seed=0,
+                 out_bias=math.atanh((math.log(INIT_REL_SHIFT) - H_MID) / H_AMP)):
+        rng = np.random.default_rng(seed)
+        init = {
+            "W1": rng.standard_normal((d0, hidden)) * math.sqrt(2.0 / d0),
+            "b1": np.zeros(hidden),
+            "W2": rng.standard_normal((hidden, hidden)) * math.sqrt(2.0 / hidden),
+            "b2": np.zeros(hidden),
+            "W3": rng.standard_normal((hidden, d3)) * math.sqrt(1.0 / hidden) * 0.1,
+            "b3": np.full(d3, out_bias),
+        }
+        self.d0, self.h, self.d3 = d0, hidden, d3
+        DA = capi.DeviceArray
+        self.p = {k: DA.from_numpy(v.astype(np.float32)) for k, v in init.items()}
+        self.g = {k: DA(v.shape, np.float32) for k, v in init.items()}
+        self.m = {k: DA(v.shape, np.float32) for k, v in init.items()}
+        self.v = {k: DA(v.shape, np.float32) for k, v in init.items()}
+        for k in self.NAMES:
+            self.m[k].zero()
+            self.v[k].zero()
+        self.wb = {k: DA(init[k].shape, np.uint16) for k in ("W1", "W2", "W3")}
+        for k in self.wb:
+            self._cast(k)
+        self.t = 0
+        self.fmean = np.zeros(d0)
+        self.fstd = np.ones(d0)
+
+    def _cast(self, k):
+        capi.check(capi.lib().kf_cast_bf16(self.p[k].ptr, int(np.prod(self.p[k].shape)),
+                                           self.wb[k].ptr, None), "kf_cast_bf16")
+
+    def set_normalisation(self, feats):
+        # features that barely vary (most descriptor means of conformers of one
+        # molecule) get a floor on their scale, so held-out inputs stay O(1)
+        sd = feats.std(0)
+        self.fmean = feats.mean(0)
+        self.fstd = sd + 1e-2 * sd.mean() + 1e-12
+
+    def normalised(self, feats):
+        return ((feats - self.fmean) / self.fstd).astype(np.float32)
+
+    def forward(self, Xf):
+        """Xf: DeviceArray (B, d0) float32 -> (Y, Z1, Z2) device arrays."""
+        B = Xf.shape[0]
+        DA = capi.DeviceArray
+        Z1, Z2, Y = DA((B, self.h), np.float32), DA((B, self.h), np.float32), DA((B, self.d3), np.float32)
+        capi.check(capi.lib().kf_mlp_forward(Xf.ptr, B, self.d0, self.h, self.d3,
+                                             self.wb["W1"].ptr, self.p["b1"].ptr,
+                                             self.wb["W2"].ptr, self.p["b2"].ptr,
+                                             self.wb["W3"].ptr, self.p["b3"].ptr, Z1.ptr, Z2.ptr,
+                                             Y.ptr, None), "kf_mlp_forward")
+        return Y, Z1, Z2
+
+    def backward(self, Xf, Z1, Z2, dY):
+        capi.check(capi.lib().kf_mlp_backward(Xf.ptr, Xf.shape[0], self.d0, self.h, self.d3,
+                                              self.wb["W1"].ptr, self.wb["W2"].ptr,
+                                              self.wb["W3"].ptr, Z1.ptr, Z2.ptr, dY.ptr,
+                                              self.g["W1"].ptr, self.g["b1"].ptr,
+                                              self.g["W2"].ptr, self.g["b2"].ptr,
+                                              self.g["W3"].ptr, self.g["b3"].ptr, None),
+                   "kf_mlp_backward")
+
+    def adam_step(self, lr=0.003, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.t += 1
+        for k in self.NAMES:
+            wb = self.wb[k].ptr if k in self.wb else None
+            capi.check(capi.lib().kf_mlp_adam(self.p[k].ptr, self.g[k].ptr, self.m[k].ptr,
+                                              self.v[k].ptr, int(np.prod(self.p[k].shape)), lr,
+                                              beta1, beta2, eps, self.t, wb, None), "kf_mlp_adam")
+
+
+def log_shift_to_shift(Y, dscale):
+    """s = d * exp(H_MID + H_AMP tanh(Y)) (host-side elementwise on a (B, bs) array)."""
+    return dscale * np.exp(H_MID + H_AMP * np.tanh(Y.astype(np.float64)))
+
+
+def shift_grad_to_output(gs, shift, Y):
+    """d loss / d Y from d loss / d s through s = d exp(H_MID + H_AMP tanh(Y))."""
+    t = np.tanh(Y.astype(np.float64))
+    return gs * shift * H_AMP * (1.0 - t * t)
+
+
+def factor(sys_: DeviceSystem, lam, shift=None):
+    """Block inverses (nm, bs, bs) of K_bb + lambda + diag(shift) on the device."""
+    inv = capi.DeviceArray((sys_.nm, sys_.bs, sys_.bs))
+    info = capi.DeviceArray(sys_.nm, np.int32)
+    dshift = capi.DeviceArray.from_numpy(np.ascontiguousarray(shift, np.float64)) if shift is not None else None
+    capi.check(capi.lib().kf_block_factor(sys_.blocks.ptr, sys_.nm, sys_.bs, sys_.dN.ptr, sys_.ma,
+                                          lam, dshift.ptr if dshift else None, inv.ptr, info.ptr,
+                                          None), "kf_block_factor")
+    bad = info.numpy()
+    if np.any(bad):
+        raise RuntimeError(f"block factor failed for {int(np.count_nonzero(bad))} blocks")
+    return inv
+
+
+def unrolled_loss_grad(sys_: DeviceSystem, lam, inv, steps=10):
+    """(loss, d loss / d shift (nm, bs)) with the explicit RFP operator."""
+    g = capi.DeviceArray((sys_.nm, sys_.bs))
+    g.zero()
+    loss = ctypes.c_double(0.0)
+    capi.check(capi.lib().kf_pcg_unrolled_grad_rfp(sys_.arf.ptr, sys_.n, lam, inv.ptr, sys_.nm,
+                                                   sys_.bs, sys_.dN.ptr, sys_.ma, sys_.y.ptr,
+                                                   steps, ctypes.byref(loss), g.ptr, None),
+               "kf_pcg_unrolled_grad_rfp")
+    return loss.value, g.numpy()
+
+
+def pcg(sys_: DeviceSystem, lam, inv=None, tol=1e-8, max_iter=500, matrix_free=True):
+    """PCG from x = 0; returns (alpha device array, iterations, true relative residual,
+    recurrence residual history)."""
+    x = capi.DeviceArray(sys_.n)
+    it, rel = ctypes.c_int(0), ctypes.c_double(0.0)
+    hist = np.zeros(max_iter + 1)
+    hp = hist.ctypes.data_as(ctypes.c_void_p)
+    if matrix_free:
+        rc = capi.lib().kf_ef_pcg_matfree(sys_.X.ptr, sys_.dX.ptr, sys_.dQ.ptr, sys_.dN.ptr,
+                                          sys_.nm, sys_.ma, sys_.rep, sys_.sigma, lam,
+                                          inv.ptr if inv is not None else None, sys_.bs,
+                                          sys_.y.ptr, x.ptr, tol, max_iter, ctypes.byref(it),
+                                          ctypes.byref(rel), hp, None)
+    else:
+        rc = capi.lib().kf_ef_pcg_rfp(sys_.arf.ptr, sys_.n, lam,
+                                      inv.ptr if inv is not None else None, sys_.nm, sys_.bs,
+                                      sys_.dN.ptr, sys_.ma, sys_.y.ptr, x.ptr, tol, max_iter,
+                                      ctypes.byref(it), ctypes.byref(rel), hp, None)
+    capi.check(rc, "PCG")
+    return x, it.value, rel.value, hist[: it.value + 1]
+
+
+def train(net: ShiftNet, systems, lam, epochs=10, steps=10, lr=0.003, log=None):
+    """Adam on the mean unrolled loss over `systems` (each with an explicit RFP).
+    Returns the per-epoch mean loss."""
+    feats = [s.features(lam) for s in systems]
+    if net.t == 0:
+        net.set_normalisation(np.concatenate(feats))
+    dfe = [capi.DeviceArray.from_numpy(net.normalised(f)) for f in feats]
+    hist = []
+    for ep in range(epochs):
+        tot = 0.0
+        for s, xf in zip(systems, dfe):
+            Y, Z1, Z2 = net.forward(xf)
+            h = Y.numpy()
+            shift = log_shift_to_shift(h, s.dscale)
+            inv = factor(s, lam, shift)
+            loss, gs = unrolled_loss_grad(s, lam, inv, steps)
+            tot += loss
+            dY = (shift_grad_to_output(gs, shift, h) / len(systems)).astype(np.float32)
+            net.backward(xf, Z1, Z2, capi.DeviceArray.from_numpy(dY))
+            net.adam_step(lr)
+        hist.append(tot / len(systems))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: mean unrolled loss {hist[-1]:.4f}")
+    return hist
+
+
+def predicted_shift(net: ShiftNet, sys_: DeviceSystem, lam):
+    xf = capi.DeviceArray.from_numpy(net.normalised(sys_.features(lam)))
+    Y, _, _ = net.forward(xf)
+    return log_shift_to_shift(Y.numpy(), sys_.dscale)

@@ -1,0 +1,177 @@
+"""GPU: the energy+force PCG pieces and the learned block-Jacobi correction vs the NumPy
+oracle (oracle/learned_pcg.py, itself pinned by finite differences on the CPU).
+
+Bars: FP64 block data and solves to 1e-8..1e-10 relative; the unrolled-PCG loss to 1e-9
+absolute and its gradient to 1e-6 relative; the BF16-weight MLP (FP32 accumulation)
+forward to 2e-5 and its gradients to 1e-4 relative (summation order differs)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import learned_pcg as lp
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def conformer_set(n, seed):
+    from kernelforge_amd import synthetic
+
+    base, z = synthetic.aspirin_base()
+    return synthetic.conformers(base, z, n, seed)
+
+
+@pytest.fixture(scope="module")
+def small():
+    """6 conformers: the device system and the oracle's dense full kernel."""
+    from kernelforge_amd import learned_pcg as L
+
+    C, Z, E, F = conformer_set(6, 11)
+    s = L.DeviceSystem(C, Z, E, F)
+    Q = np.tile(Z, (6, 1)).astype(np.int32)
+    N = np.full(6, 21, np.int32)
+    X, dX = oracle.fchl19_batch(C, Q, N, grad=True)
+    K = oracle.full_symm(X, dX, Q, N, 2.0)
+    y = np.concatenate([E, -F.reshape(-1)])
+    return s, K, y, N
+
+
+def test_diag_blocks(small):
+    s, K, y, N = small
+    ref = lp.diag_blocks(K, N, 21)
+    assert rel_err(s.blocks.numpy(), ref) < 1e-10
+
+
+def test_block_factor_and_apply(small):
+    from kernelforge_amd import capi
+    from kernelforge_amd import learned_pcg as L
+
+    s, K, y, N = small
+    shift = 1e-3 * s.dscale * (1.0 + np.random.default_rng(0).random(s.dscale.shape))
+    inv = L.factor(s, 1e-3, shift)
+    ref = lp.block_inverses(lp.diag_blocks(K, N, 21), N, 21, 1e-3, shift)
+    r = np.random.default_rng(1).standard_normal(s.n)
+    dr, dz = capi.DeviceArray.from_numpy(r), capi.DeviceArray(s.n)
+    capi.check(capi.lib().kf_block_apply(inv.ptr, s.nm, s.bs, s.dN.ptr, s.ma, dr.ptr, dz.ptr,
+                                         None), "kf_block_apply")
+    assert rel_err(dz.numpy(), lp.apply_blocks(ref, N, 21, r)) < 1e-8
+
+
+def test_pcg_rfp_and_matrix_free(small):
+    from kernelforge_amd import learned_pcg as L
+
+    s, K, y, N = small
+    lam = 0.1
+    s.build_rfp()
+    inv = L.factor(s, lam)
+    exact = np.linalg.solve(K + lam * np.eye(len(y)), y)
+    x1, it1, rel1, h1 = L.pcg(s, lam, inv, tol=1e-10, max_iter=4000, matrix_free=False)
+    x2, it2, rel2, h2 = L.pcg(s, lam, inv, tol=1e-10, max_iter=4000, matrix_free=True)
+    assert rel1 < 1e-9 and rel2 < 1e-9
+    assert rel_err(x1.numpy(), exact) < 1e-6 and rel_err(x2.numpy(), exact) < 1e-6
+    ref_inv = lp.block_inverses(lp.diag_blocks(K, N, 21), N, 21, lam)
+    _, it0, _ = lp.pcg(K + lam * np.eye(len(y)), y, lambda r: lp.apply_blocks(ref_inv, N, 21, r),
+                       tol=1e-10, max_iter=4000)
+    assert abs(it1 - it0) <= max(3, it0 // 20) and abs(it2 - it0) <= max(3, it0 // 20)
+
+
+def test_unrolled_grad_vs_oracle(small):
+    from kernelforge_amd import learned_pcg as L
+
+    s, K, y, N = small
+    lam = 1e-2
+    s.build_rfp()
+    shift = 1e-3 * s.dscale
+    inv = L.factor(s, lam, shift)
+    loss, g = L.unrolled_loss_grad(s, lam, inv, steps=10)
+    ref_inv = lp.block_inverses(lp.diag_blocks(K, N, 21), N, 21, lam, shift)
+    loss_o, g_o = lp.unrolled_loss_grad(K + lam * np.eye(len(y)), y, ref_inv, N, 21, 10)
+    assert abs(loss - loss_o) < 1e-9
+    assert rel_err(g, g_o) < 1e-6
+
+
+def test_mlp_forward_backward_vs_oracle():
+    from kernelforge_amd import capi
+    from kernelforge_amd import learned_pcg as L
+
+    rng = np.random.default_rng(3)
+    B, d0, d3 = 37, 722, 64
+    net = L.ShiftNet(d0, d3, seed=5)
+    p = {k: net.p[k].numpy() for k in net.NAMES}
+    p["b1"] = rng.standard_normal(128).astype(np.float32) * 0.1
+    net.p["b1"] = capi.DeviceArray.from_numpy(p["b1"])
+    X = rng.standard_normal((B, d0)).astype(np.float32)
+    dX = capi.DeviceArray.from_numpy(X)
+    Y, Z1, Z2 = net.forward(dX)
+    Yo, cache = lp.mlp_forward(X, p, bf16=True)
+    assert rel_err(Y.numpy(), Yo) < 2e-5
+    assert rel_err(Z1.numpy(), cache[1]) < 2e-5 and rel_err(Z2.numpy(), cache[2]) < 2e-5
+    G = rng.standard_normal((B, d3)).astype(np.float32)
+    net.backward(dX, Z1, Z2, capi.DeviceArray.from_numpy(G))
+    go = lp.mlp_backward(cache, G, p, bf16=True)
+    for k in net.NAMES:
+        assert rel_err(net.g[k].numpy(), go[k]) < 1e-4, k
+    # the device BF16 copies are the oracle's rounding of the FP32 masters
+    for k in ("W1", "W2", "W3"):
+        wb = net.wb[k].numpy().astype(np.uint32) << 16
+        np.testing.assert_array_equal(wb.view(np.float32), lp.to_bf16(p[k]))
+
+
+def test_adam_step_matches_numpy():
+    from kernelforge_amd import capi
+
+    rng = np.random.default_rng(4)
+    n = 1000
+    p0, g = rng.standard_normal(n).astype(np.float32), rng.standard_normal(n).astype(np.float32)
+    m0 = rng.standard_normal(n).astype(np.float32) * 0.1
+    v0 = np.abs(rng.standard_normal(n)).astype(np.float32) * 0.1
+    dp, dg, dm, dv = (capi.DeviceArray.from_numpy(a) for a in (p0, g, m0, v0))
+    pb = capi.DeviceArray(n, np.uint16)
+    t, lr, b1, b2, eps = 3, 0.01, 0.9, 0.999, 1e-8
+    capi.check(capi.lib().kf_mlp_adam(dp.ptr, dg.ptr, dm.ptr, dv.ptr, n, lr, b1, b2, eps, t,
+                                      pb.ptr, None), "kf_mlp_adam")
+    m = b1 * m0 + (1 - b1) * g
+    v = b2 * v0 + (1 - b2) * g * g
+    p = p0 - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    assert rel_err(dp.numpy(), p) < 1e-6
+    np.testing.assert_array_equal((pb.numpy().astype(np.uint32) << 16).view(np.float32),
+                                  lp.to_bf16(dp.numpy()))
+
+
+def test_training_reduces_unrolled_loss():
+    """Ablation: the trained network vs plain block-Jacobi (zero shift) and vs its own
+    initialisation, on the 10-step unrolled PCG loss of the systems it trains on; the
+    held-out system's numbers are printed (with 6 x 8 conformers the network does not
+    generalise; the workload-scale measurement is reported by bench.py)."""
+    from kernelforge_amd import learned_pcg as L
+
+    lam = 1e-2
+    C, Z, E, F = conformer_set(56, 7)
+    systems = []
+    for k in range(7):
+        sl = slice(8 * k, 8 * k + 8)
+        s = L.DeviceSystem(C[sl], Z, E[sl], F[sl])
+        s.build_rfp()
+        systems.append(s)
+    train, held = systems[:6], systems[6]
+
+    def mean_loss(net_or_none, group):
+        out = []
+        for s in group:
+            shift = None if net_or_none is None else L.predicted_shift(net_or_none, s, lam)
+            out.append(L.unrolled_loss_grad(s, lam, L.factor(s, lam, shift), 10)[0])
+        return float(np.mean(out))
+
+    net = L.ShiftNet(722, 64, seed=0)
+    net.set_normalisation(np.concatenate([s.features(lam) for s in train]))
+    zero_train, init_train = mean_loss(None, train), mean_loss(net, train)
+    zero_held, init_held = mean_loss(None, [held]), mean_loss(net, [held])
+    hist = L.train(net, train, lam, epochs=15, steps=10, lr=0.003)
+    trained_train, trained_held = mean_loss(net, train), mean_loss(net, [held])
+    print(f"unrolled log-residual (10 steps), train: zero shift {zero_train:.4f}, "
+          f"initial net {init_train:.4f}, trained {trained_train:.4f}; held-out: zero shift "
+          f"{zero_held:.4f}, initial {init_held:.4f}, trained {trained_held:.4f}; "
+          f"epochs {np.round(hist, 4).tolist()}")
+    assert trained_train < init_train - 0.02
+    assert trained_train < zero_train - 0.02
