@@ -34,6 +34,7 @@ METRIC = ("KRR solve s @≤1e-6 rel err vs FP64: plain vs mixed+learned PCG; ker
 PEAK_F64_MFMA_TFLOPS = 78.6
 PEAK_BF16_MFMA_TFLOPS = 2500.0
 SIGMA, LAMBDA = 2.0, 1e-8
+ENERGY_REL_BAR = 1e-6
 ELEMENTS = (1, 6, 7, 8, 16)
 
 
@@ -95,6 +96,257 @@ def cpu_baseline(C, Q, N, y, sample, full_rows_sq, n_full):
     }
 
 
+EF_LAMBDA = 1e-10
+
+
+def ef_pair_flop(na=21, matched=161, rep=720):
+    """Algorithmic FLOP of one molecule pair of the full E+F kernel (EE, EF, FE, FF):
+    three projections and a rank-1 update per matched atom pair plus the T_i contraction."""
+    nc = 3 * na
+    return matched * (6 * rep * nc + 2 * nc * nc) + na * 2 * nc * nc * rep
+
+
+def ef_cpu_baseline(C, Z, E, F, sample, n_full):
+    """Oracle E+F fit (descriptors + Jacobians, full RFP kernel, dpftrf/dpftrs) on
+    `sample` conformers, scaled to n_full by n (descriptors), pairs (kernel), n^3 (solve)."""
+    import oracle
+    from threadpoolctl import threadpool_limits
+
+    cores = cpu_threads()
+    Q = np.tile(Z, (sample, 1)).astype(np.int32)
+    Nn = np.full(sample, len(Z), np.int32)
+    with threadpool_limits(limits=cores):
+        oracle.fchl19_batch(C[:2], Q[:2], Nn[:2], grad=True)
+        t0 = time.perf_counter()
+        X, dX = oracle.fchl19_batch(C[:sample], Q, Nn, grad=True)
+        t1 = time.perf_counter()
+        arf = oracle.full_symm_rfp(X, dX, Q, Nn, SIGMA)
+        t2 = time.perf_counter()
+        y = np.concatenate([E[:sample], -F[:sample].reshape(-1)])
+        oracle.cho_solve_rfp(arf, y, EF_LAMBDA)
+        t3 = time.perf_counter()
+    big_s, big_f = sample * (1 + 3 * len(Z)), n_full * (1 + 3 * len(Z))
+    s_desc = (t1 - t0) * n_full / sample
+    s_kern = (t2 - t1) * (n_full * (n_full + 1)) / (sample * (sample + 1))
+    s_chol = (t3 - t2) * (big_f / big_s) ** 3
+    return {
+        "value": round(s_desc + s_kern + s_chol, 2), "unit": "s", "cores": cores, "kind": "port",
+        "sample": (f"oracle on {sample} of {n_full} conformers: descriptors+Jacobians "
+                   f"{t1 - t0:.2f} s, full E+F RFP kernel {t2 - t1:.2f} s, dpftrf+dpftrs "
+                   f"{t3 - t2:.3f} s (order {big_s}); scaled by n, pairs and n^3: "
+                   f"{s_desc:.1f} + {s_kern:.1f} + {s_chol:.1f} s"),
+    }
+
+
+def ef_leg(args, L):
+    """Plain energy+force fit: 500 C9H8O4 conformers (seed 2), descriptors + Jacobians,
+    full E+F kernel (order 32,000) in FP64 RFP, FP64 RFP Cholesky (lambda 1e-10), solve;
+    held-out conformers predicted through alpha_desc + the full matvec."""
+    from kernelforge_amd import capi, synthetic
+
+    nconf, ntest = args.ef_conformers, 100
+    base, z = synthetic.aspirin_base()
+    C, Z, E, F = synthetic.conformers(base, z, nconf, 2)
+    Ct, _, Et, Ft = synthetic.conformers(base, z, ntest, 102)
+    ma, rs = len(Z), capi.rep_size()
+    big = nconf * (1 + 3 * ma)
+    DA = capi.DeviceArray
+    Q = np.tile(Z, (nconf, 1)).astype(np.int32)
+    Nn = np.full(nconf, ma, np.int32)
+    dC, dQ, dN = DA.from_numpy(C), DA.from_numpy(Q), DA.from_numpy(Nn)
+    X, dX = DA((nconf, ma, rs)), DA((nconf, ma, rs, 3 * ma))
+    arf = DA(big * (big + 1) // 2)
+    y = np.concatenate([E, -F.reshape(-1)])
+    dy, alpha = DA.from_numpy(y), DA(big)
+    log(f"E+F fit: {nconf} conformers, order {big}, RFP {arf.nbytes / 1e9:.2f} GB, "
+        f"dX {dX.nbytes / 1e9:.2f} GB")
+
+    def step():
+        capi.fchl19(dC.ptr, dQ.ptr, dN.ptr, nconf, ma, X.ptr, dX.ptr)
+        capi.local_kernel_full_symm_rfp(X.ptr, dX.ptr, dQ.ptr, dN.ptr, nconf, ma, rs, SIGMA,
+                                        arf.ptr)
+        capi.rfp_potrf(capi.KF_PREC_F64, arf.ptr, big, EF_LAMBDA)
+        alpha.copy_from(dy)
+        capi.rfp_potrs(capi.KF_PREC_F64, arf.ptr, big, alpha.ptr, 1)
+
+    for _ in range(args.warmup):
+        step()
+        capi.synchronize()
+    L.kf_timer_reset()
+    L.kf_timer_enable(1)
+    steps = max(1, min(args.steps, 2))
+    capi.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        capi.synchronize()
+        log(f"E+F fit: step {k + 1}/{steps} done")
+    sec = (time.perf_counter() - t0) / steps
+    L.kf_timer_enable(0)
+    tm = {k: capi.timer_read(v) for k, v in (("fchl19", capi.TM_FCHL19), ("kernel", capi.TM_FULL),
+                                             ("potrf", capi.TM_POTRF), ("potrs", capi.TM_POTRS))}
+    ms = {k: (t / c if c else None) for k, (t, c) in tm.items()}
+    # held-out predictions (after the timed region)
+    a = alpha.numpy()
+    Qt = np.tile(Z, (ntest, 1)).astype(np.int32)
+    Nt = np.full(ntest, ma, np.int32)
+    dCt, dQt, dNt = DA.from_numpy(Ct), DA.from_numpy(Qt), DA.from_numpy(Nt)
+    Xt, dXt = DA((ntest, ma, rs)), DA((ntest, ma, rs, 3 * ma))
+    capi.fchl19(dCt.ptr, dQt.ptr, dNt.ptr, ntest, ma, Xt.ptr, dXt.ptr)
+    ad, aF, aE = DA((nconf, ma, rs)), DA.from_numpy(a[nconf:]), DA.from_numpy(a[:nconf])
+    capi.local_alpha_desc(dX.ptr, dQ.ptr, dN.ptr, nconf, ma, rs, aF.ptr, ad.ptr)
+    Ep, Fp = DA(ntest), DA(ntest * 3 * ma)
+    capi.local_full_matvec(Xt.ptr, dXt.ptr, X.ptr, ad.ptr, aE.ptr, dQt.ptr, dQ.ptr, dNt.ptr,
+                           dN.ptr, ntest, nconf, ma, ma, rs, SIGMA, Ep.ptr, Fp.ptr)
+    capi.synchronize()
+    e_mae = float(np.mean(np.abs(Ep.numpy() - Et)))
+    f_mae = float(np.mean(np.abs(-Fp.numpy() - Ft.reshape(-1))))
+    flop = ef_pair_flop(ma) * nconf * (nconf + 1) / 2
+    kt = flop / (ms["kernel"] * 1e-3) / 1e12
+    out = {
+        "workload": (f"energy+force KRR fit, {nconf} synthetic C9H8O4 conformers (seed 2): "
+                     "descriptors + Jacobians, full E+F kernel in FP64 RFP (order "
+                     f"{big}), FP64 RFP Cholesky lambda 1e-10, solve"),
+        "s_per_fit": round(sec, 4),
+        "breakdown_ms": {k: (round(v, 3) if v is not None else None) for k, v in ms.items()},
+        "roofline": {"bound": "mfma", "achieved": round(kt, 3), "peak": PEAK_F64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(kt / PEAK_F64_MFMA_TFLOPS, 4),
+                     "traffic": None, "kernel": "deriv_prep_kernel + deriv_ff_kernel "
+                     "<FULL_SYMM_RFP>", "flop_per_launch": flop},
+        "test_mae": {"energy_ha": round(e_mae, 6), "force_ha_per_a": round(f_mae, 6),
+                     "n_test_conformers": ntest},
+    }
+    log(f"E+F fit {sec:.3f} s/fit, breakdown {out['breakdown_ms']}, kernel {kt:.1f} TFLOP/s, "
+        f"test MAE E {e_mae:.4g} F {f_mae:.4g}")
+    if args.ef_cpu_sample > 0:
+        log(f"E+F CPU baseline on {args.ef_cpu_sample} conformers ...")
+        out["cpu_baseline"] = ef_cpu_baseline(C, Z, E, F, args.ef_cpu_sample, nconf)
+        log(f"E+F CPU baseline {out['cpu_baseline']['value']} s (scaled)")
+    return out
+
+
+def learned_pcg_leg(args, L):
+    """Energy+force fit of 2,000 C9H8O4 conformers (seed 3, order 128,000, lambda 1e-10):
+    the shift network is trained on kernel systems of random 250-conformer subsets, then
+    PCG with the matrix-free full matvec runs with the learned block-Jacobi correction and
+    with plain block-Jacobi (fallback), both capped; FP64 RFP Cholesky on the full system
+    is the direct solve and the judge of the error."""
+    from kernelforge_amd import capi, synthetic
+    from kernelforge_amd import learned_pcg as LP
+
+    lam, nconf, ntest = EF_LAMBDA, args.pcg_conformers, 100
+    base, z = synthetic.aspirin_base()
+    C, Z, E, F = synthetic.conformers(base, z, nconf, 3)
+    Ct, _, Et, Ft = synthetic.conformers(base, z, ntest, 103)
+    t0 = time.perf_counter()
+    full = LP.DeviceSystem(C, Z, E, F)
+    capi.synchronize()
+    log(f"learned PCG: {nconf} conformers, order {full.n}; descriptors + Jacobians + "
+        f"diagonal blocks {time.perf_counter() - t0:.2f} s")
+    # ---- training systems: random subsets of the training conformers ----
+    rng = np.random.default_rng(3)
+    systems = []
+    t0 = time.perf_counter()
+    for k in range(args.pcg_systems):
+        idx = np.sort(rng.choice(nconf, args.pcg_subset, replace=False))
+        s_ = LP.DeviceSystem(C[idx], Z, E[idx], F[idx])
+        s_.build_rfp()
+        s_.drop_jacobians()
+        systems.append(s_)
+    capi.synchronize()
+    t_build = time.perf_counter() - t0
+    log(f"learned PCG: {len(systems)} training systems of {args.pcg_subset} conformers built "
+        f"in {t_build:.1f} s")
+    net = LP.ShiftNet(722, full.bs, seed=0)
+    t0 = time.perf_counter()
+    hist = LP.train(net, systems, lam, epochs=args.pcg_epochs, steps=10, log=log)
+    capi.synchronize()
+    t_train = time.perf_counter() - t0
+    zero_loss = float(np.mean([LP.unrolled_loss_grad(s_, lam, LP.factor(s_, lam), 10)[0]
+                               for s_ in systems[:8]]))
+    trained_loss = float(np.mean([LP.unrolled_loss_grad(
+        s_, lam, LP.factor(s_, lam, LP.predicted_shift(net, s_, lam)), 10)[0]
+        for s_ in systems[:8]]))
+    systems = None  # release the training kernels before the full-size solves
+    # ---- PCG on the full system, matrix-free ----
+    res = {}
+    for name, shift in (("learned", LP.predicted_shift(net, full, lam)), ("block_jacobi", None)):
+        inv = LP.factor(full, lam, shift)
+        capi.synchronize()
+        t0 = time.perf_counter()
+        x, it, rel, h = LP.pcg(full, lam, inv, tol=1e-8, max_iter=args.pcg_iters,
+                               matrix_free=True)
+        capi.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"alpha": x.numpy(), "iters": it, "rel_residual": rel, "seconds": dt,
+                     "history_every_10": [float(v) for v in h[::10]]}
+        log(f"learned PCG: {name}: {it} iterations, true residual {rel:.3e}, {dt:.2f} s "
+            f"({dt / max(it, 1) * 1e3:.1f} ms/iteration)")
+    # ---- direct FP64 RFP Cholesky on the full system (the judge) ----
+    alpha = capi.DeviceArray(full.n)
+    t0 = time.perf_counter()
+    full.build_rfp()
+    capi.rfp_potrf(capi.KF_PREC_F64, full.arf.ptr, full.n, lam)
+    alpha.copy_from(full.y)
+    capi.rfp_potrs(capi.KF_PREC_F64, full.arf.ptr, full.n, alpha.ptr, 1)
+    capi.synchronize()
+    t_chol = time.perf_counter() - t0
+    a_chol = alpha.numpy()
+    full.free_rfp()
+    log(f"learned PCG: FP64 RFP Cholesky (kernel build + potrf + potrs) {t_chol:.2f} s")
+    # ---- held-out predictions of each solution ----
+    test = LP.DeviceSystem(Ct, Z, Et, Ft)
+
+    def predict(a):
+        DA = capi.DeviceArray
+        ad, aF, aE = DA((nconf, full.ma, full.rep)), DA.from_numpy(a[nconf:]), DA.from_numpy(a[:nconf])
+        capi.local_alpha_desc(full.dX.ptr, full.dQ.ptr, full.dN.ptr, nconf, full.ma, full.rep,
+                              aF.ptr, ad.ptr)
+        Ep, Fp = DA(ntest), DA(ntest * 3 * full.ma)
+        capi.local_full_matvec(test.X.ptr, test.dX.ptr, full.X.ptr, ad.ptr, aE.ptr, test.dQ.ptr,
+                               full.dQ.ptr, test.dN.ptr, full.dN.ptr, ntest, nconf, full.ma,
+                               full.ma, full.rep, SIGMA, Ep.ptr, Fp.ptr)
+        capi.synchronize()
+        return Ep.numpy(), -Fp.numpy()
+
+    Ec, Fc = predict(a_chol)
+    out_sol = {"cholesky_fp64": {"seconds": round(t_chol, 3),
+                                 "test_mae_energy": float(np.mean(np.abs(Ec - Et))),
+                                 "test_mae_force": float(np.mean(np.abs(Fc - Ft.reshape(-1))))}}
+    for name, r in res.items():
+        Ep, Fp = predict(r["alpha"])
+        out_sol[name + "_pcg"] = {
+            "seconds": round(r["seconds"], 3), "iterations": r["iters"],
+            "rel_residual": r["rel_residual"],
+            "energy_rel_err_vs_fp64": float(np.max(np.abs(Ep - Ec) / np.abs(Ec))),
+            "force_rel_err_vs_fp64": float(np.max(np.abs(Fp - Fc)) / np.max(np.abs(Fc))),
+            "test_mae_energy": float(np.mean(np.abs(Ep - Et))),
+            "test_mae_force": float(np.mean(np.abs(Fp - Ft.reshape(-1)))),
+            "residual_history_every_10": r["history_every_10"],
+        }
+    meets = [k for k, v in out_sol.items() if k == "cholesky_fp64"
+             or v["energy_rel_err_vs_fp64"] <= ENERGY_REL_BAR]
+    best = min(meets, key=lambda k: out_sol[k]["seconds"])
+    return {
+        "metric": METRIC, "value": out_sol[best]["seconds"], "unit": "s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(out_sol[best]["seconds"] * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"energy+force fit of {nconf} C9H8O4 conformers (seed 3), "
+                                f"order {full.n}, lambda 1e-10: learned block-Jacobi PCG "
+                                "(matrix-free full matvec) vs block-Jacobi PCG vs FP64 RFP "
+                                "Cholesky; value = fastest solve within 1e-6 relative energy "
+                                "error of the FP64 Cholesky solution"),
+                   "selected": best, "pcg_tol": 1e-8, "pcg_iteration_cap": args.pcg_iters},
+        "solves": out_sol,
+        "training": {"systems": args.pcg_systems, "conformers_per_system": args.pcg_subset,
+                     "epochs": args.pcg_epochs, "build_s": round(t_build, 2),
+                     "train_s": round(t_train, 2), "epoch_mean_unrolled_loss": hist,
+                     "unrolled_loss_first8_zero_shift": zero_loss,
+                     "unrolled_loss_first8_trained": trained_loss},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +356,18 @@ def main(argv=None):
     ap.add_argument("--n-test", type=int, default=2000)
     ap.add_argument("--cpu-sample", type=int, default=3000,
                     help="molecules in the CPU-baseline sample (0 skips it)")
+    ap.add_argument("--ef-conformers", type=int, default=500,
+                    help="conformers of the energy+force fit leg (0 skips it)")
+    ap.add_argument("--ef-cpu-sample", type=int, default=12,
+                    help="conformers in the energy+force CPU-baseline sample (0 skips it)")
+    ap.add_argument("--workload", choices=["energy", "learned-pcg"], default="energy",
+                    help="energy: the 20k energy fit (+ the 500-conformer E+F fit); "
+                         "learned-pcg: the 2,000-conformer learned-preconditioner E+F fit")
+    ap.add_argument("--pcg-conformers", type=int, default=2000)
+    ap.add_argument("--pcg-systems", type=int, default=64)
+    ap.add_argument("--pcg-subset", type=int, default=250)
+    ap.add_argument("--pcg-epochs", type=int, default=6)
+    ap.add_argument("--pcg-iters", type=int, default=100)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="fp64: plain path; mixed: BF16x3 kernel + FP32 factor + PCG; both: "
                          "time both, report the mixed fit with its error vs the FP64 fit")
@@ -118,6 +382,11 @@ def main(argv=None):
     L = capi.lib()
     name = ctypes_name(L)
     log(f"device {name}")
+    if args.workload == "learned-pcg":
+        out = learned_pcg_leg(args, L)
+        out["device"] = name
+        print(json.dumps(out), flush=True)
+        return 0
 
     ntr, nte = args.n_train, args.n_test
     t0 = time.perf_counter()
@@ -189,17 +458,9 @@ def main(argv=None):
             log(f"mixed path failed: {exc}")
             if not res:
                 raise
-    head = "mixed" if "mixed" in res else "fp64"
-    sec_per_fit, avg_ms = res[head]
-
     rows = label_rows(Q[:ntr], N[:ntr])
     rows_sq = sum(v * v for v in rows.values())
     kernel_flop = rs * rows_sq  # symmetric half of 2 * R_e^2 * rep, summed over labels
-    if head == "mixed":  # three BF16 products per contraction term
-        roof_flop, peak, kname = 3 * kernel_flop, PEAK_BF16_MFMA_TFLOPS, "local_bf16x3_kernel<SYMM_RFP>"
-    else:
-        roof_flop, peak, kname = kernel_flop, PEAK_F64_MFMA_TFLOPS, "local_scalar_f64_kernel<SYMM_RFP>"
-    achieved = roof_flop / (avg_ms["kernel"] * 1e-3) / 1e12
 
     # test predictions and errors (after the timed regions; not part of a step)
     Xte = DA((nte, ma, rs))
@@ -236,6 +497,21 @@ def main(argv=None):
                 kernel_rel_err_test_block=float(np.max(np.abs(Kte32.numpy() - Kte_h)) /
                                                 np.max(np.abs(Kte_h))),
                 speedup_vs_fp64=res["fp64"][0] / res["mixed"][0])
+    # The metric is the fit time at <= 1e-6 relative energy error vs FP64: the mixed
+    # path is the head only when its test energies meet that bar (and it is faster).
+    head = "fp64" if "fp64" in res else "mixed"
+    if ("mixed" in res and "fp64" in res
+            and accuracy.get("energy_rel_err_max", 1.0) <= ENERGY_REL_BAR
+            and res["mixed"][0] < res["fp64"][0]):
+        head = "mixed"
+    if accuracy:
+        accuracy["meets_1e-6_bar"] = bool(accuracy.get("energy_rel_err_max", 1.0) <= ENERGY_REL_BAR)
+    sec_per_fit, avg_ms = res[head]
+    if head == "mixed":  # three BF16 products per contraction term
+        roof_flop, peak, kname = 3 * kernel_flop, PEAK_BF16_MFMA_TFLOPS, "local_bf16x3_kernel<SYMM_RFP>"
+    else:
+        roof_flop, peak, kname = kernel_flop, PEAK_F64_MFMA_TFLOPS, "local_scalar_f64_kernel<SYMM_RFP>"
+    achieved = roof_flop / (avg_ms["kernel"] * 1e-3) / 1e12
     log(f"{head} fit {sec_per_fit:.3f} s/step, kernel {avg_ms['kernel']:.1f} ms "
         f"({achieved:.1f} TFLOP/s), breakdown {avg_ms}")
     if accuracy:
@@ -291,8 +567,16 @@ def main(argv=None):
                              "kernel_tflops": round(kernel_flop / (res["fp64"][1]["kernel"] * 1e-3)
                                                     / 1e12, 3),
                              "test_mae_ha": round(mae, 5)}
+    if "mixed" in res:
+        out["mixed"] = {"s_per_fit": round(res["mixed"][0], 4),
+                        "breakdown_ms": {k: (round(v, 3) if v is not None else None)
+                                         for k, v in res["mixed"][1].items()},
+                        "kernel_tflops_bf16x3": round(3 * kernel_flop /
+                                                      (res["mixed"][1]["kernel"] * 1e-3) / 1e12, 3)}
     if accuracy:
         out["mixed_accuracy_vs_fp64"] = accuracy
+    if args.ef_conformers > 0:
+        out["energy_force_fit"] = ef_leg(args, L)
     print(json.dumps(out), flush=True)
     return 0
 

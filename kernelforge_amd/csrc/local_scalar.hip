@@ -9,15 +9,17 @@
 // gfx950 design
 //  * Atoms are packed by (label, molecule) once (local_plan.hpp), so the rows of a
 //    molecule block for one label are one contiguous range.
-//  * The OUTPUT is tiled: one workgroup owns a 32x32 block of molecule pairs (upper
+//  * The OUTPUT is tiled: one workgroup owns a 64x64 block of molecule pairs (upper
 //    triangle of blocks for the symmetric forms).  For every label it runs the
 //    rows(A-block) x rows(B-block) Gram product as 64x64 sub-tiles on FP64 MFMA
 //    (v_mfma_f64_16x16x4, 2x2 waves of 32x32), with operands staged through LDS
 //    and the next k-chunk prefetched into registers during the MFMAs.
 //  * Epilogue fused per sub-tile: squared norms (FP64), exp, then a deterministic
-//    two-stage segmented reduction in LDS (rows -> molecule columns, then molecule
-//    rows) into the workgroup's 32x32 accumulator.  No atomics; each K entry is
-//    written exactly once, straight into RFP (UPLO='U') or dense layout.
+//    two-stage segmented reduction (rows -> column-molecule runs in LDS, then row
+//    molecules into per-thread register accumulators of the 64x64 block).  No
+//    atomics; each K entry is written exactly once, straight into RFP (UPLO='U') or
+//    dense layout.  64-molecule blocks keep the 64-row label sub-tiles ~81% full on
+//    ragged molecules (32-molecule blocks: ~67%).
 #include <cmath>
 
 #include "local_mixed.hpp"
@@ -27,7 +29,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int MB = 32;       // molecules per output block
+constexpr int MB = 64;       // molecules per output block
 constexpr int TS = 64;       // sub-tile edge (packed rows)
 constexpr int KC = 16;       // k chunk staged in LDS
 constexpr int LDA = KC + 1;  // LDS pitch of operand tiles (doubles)
@@ -69,6 +71,12 @@ __device__ inline void decode_upper(int t, int nb, int &A, int &B) {
     B = a + (int)(t - ((long)a * nb - (long)a * (a - 1) / 2));
 }
 
+// Output block: 64 x 64 molecule pairs per work-group.  The block's K entries live in
+// registers: thread t owns column molecule b = t % 64 and row molecules
+// 16 (t / 64) .. +15.  Per sub-tile the exp tile is reduced in place to row-wise run
+// sums over column molecules (a run is the columns of one molecule; run index <= column
+// index, so the in-place write never overtakes the read), then each thread adds the
+// rows of its 16 row molecules for its column molecule's run.
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
@@ -78,9 +86,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     __shared__ double As[TS * LDA];
     __shared__ double Bs[TS * LDA];
     __shared__ double Es[TS * EP];
-    __shared__ double part[TS * MB];
-    __shared__ double acc[MB * MB];
-    __shared__ int rowmol[TS], colmol[TS];
+    __shared__ int rowmol[TS], colmol[TS], colrun[TS], run_of_mol[MB];
     __shared__ int mrA[MB + 1], mrB[MB + 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -95,13 +101,15 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const int a0 = A * MB, na = min(MB, nm1 - a0);
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
     const bool diag = (MODE != RECT) && (A == B);
-
-    for (int t = tid; t < MB * MB; t += NT) acc[t] = 0.0;
+    const int ob = tid & (MB - 1), oa0 = (tid / MB) * (MB / 4);
+    double acc[MB / 4];
+#pragma unroll
+    for (int k = 0; k < MB / 4; ++k) acc[k] = 0.0;
 
     for (int l = 0; l < L; ++l) {
         __syncthreads();
         if (tid <= na) mrA[tid] = mr1[(size_t)l * (nm1 + 1) + a0 + tid];
-        if (tid >= 64 && tid - 64 <= nb) mrB[tid - 64] = mr2[(size_t)l * (nm2 + 1) + b0 + tid - 64];
+        if (tid >= 128 && tid - 128 <= nb) mrB[tid - 128] = mr2[(size_t)l * (nm2 + 1) + b0 + tid - 128];
         __syncthreads();
         const int ra0 = mrA[0], ra1 = mrA[na], rb0 = mrB[0], rb1 = mrB[nb];
         if (ra1 == ra0 || rb1 == rb0) continue;
@@ -110,27 +118,32 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
             for (int sc = diag ? sr : 0; sc < nsc; ++sc) {
                 const int r0 = ra0 + sr * TS, rn = min(TS, ra1 - r0);
                 const int c0 = rb0 + sc * TS, cn = min(TS, rb1 - c0);
-                __syncthreads();  // previous sub-tile's reduction is done with part/rowmol
-                // molecule of every row / column of the sub-tile
-                if (tid < TS) {
+                __syncthreads();  // the previous sub-tile's stage 2 is done with Es
+                if (tid >= 128 && tid < 128 + MB) run_of_mol[tid - 128] = -1;
+                __syncthreads();
+                if (wave == 0) {  // molecule of every row
                     int m = -1;
-                    if (tid < rn) {
-                        const int r = r0 + tid;
+                    if (lane < rn) {
+                        const int r = r0 + lane;
                         m = 0;
                         while (mrA[m + 1] <= r) ++m;
                     }
-                    rowmol[tid] = m;
-                } else if (tid < 2 * TS) {
-                    const int t = tid - TS;
+                    rowmol[lane] = m;
+                } else if (wave == 1) {  // molecule and run of every column
                     int m = -1;
-                    if (t < cn) {
-                        const int r = c0 + t;
+                    if (lane < cn) {
+                        const int r = c0 + lane;
                         m = 0;
                         while (mrB[m + 1] <= r) ++m;
                     }
-                    colmol[t] = m;
+                    const int prev = __shfl_up(m, 1, 64);
+                    const bool head = lane < cn && (lane == 0 || prev != m);
+                    const unsigned long long heads = __ballot(head);
+                    const int run = __popcll(heads & ((2ull << lane) - 1)) - 1;
+                    colmol[lane] = m;
+                    colrun[lane] = run;
+                    if (head) run_of_mol[m] = run;
                 }
-                for (int t = tid; t < TS * MB; t += NT) part[t] = 0.0;
 
                 // ---- Gram sub-tile on MFMA ----
                 v4d c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
@@ -193,50 +206,52 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     }
                 }
                 __syncthreads();
-                // stage 1: row-wise run sums over column molecules
-                if (tid < TS && tid < rn) {
-                    const double *er = Es + tid * EP;
-                    double *pr = part + tid * MB;
-                    int cur = colmol[0];
+                // stage 1: row-wise run sums over column molecules, in place
+                if (tid < rn) {
+                    double *er = Es + tid * EP;
+                    int cur = 0;
                     double s = 0.0;
                     for (int j = 0; j < cn; ++j) {
-                        const int m = colmol[j];
-                        if (m != cur) {
-                            pr[cur] = s;
+                        const int rj = colrun[j];
+                        if (rj != cur) {
+                            er[cur] = s;
                             s = 0.0;
-                            cur = m;
+                            cur = rj;
                         }
                         s += er[j];
                     }
-                    pr[cur] = s;
+                    er[cur] = s;
                 }
                 __syncthreads();
-                // stage 2: column-molecule owner adds row runs into the accumulator
-                if (tid < MB) {
-                    for (int i = 0; i < rn; ++i) acc[rowmol[i] * MB + tid] += part[i * MB + tid];
+                // stage 2: each thread adds the rows of its row molecules for its column
+                const int run = ob < nb ? run_of_mol[ob] : -1;
+                if (run >= 0) {
+#pragma unroll
+                    for (int k = 0; k < MB / 4; ++k) {
+                        const int a = oa0 + k;
+                        if (a < na) {
+                            const int rs = max(mrA[a], r0) - r0, re = min(mrA[a + 1], r0 + rn) - r0;
+                            double s = 0.0;
+                            for (int i = rs; i < re; ++i) s += Es[i * EP + run];
+                            acc[k] += s;
+                        }
+                    }
                 }
             }
         }
     }
-    __syncthreads();
     // ---- write the block ----
-    if (MODE == SYMM_RFP) {
-        for (int t = tid; t < MB * MB; t += NT) {
-            const int b = t / MB, a = t % MB;
-            if (a < na && b < nb && (!diag || a <= b))
-                out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = acc[a * MB + b];
-        }
-    } else {
-        for (int t = tid; t < MB * MB; t += NT) {
-            const int a = t / MB, b = t % MB;
-            if (a < na && b < nb) {
-                if (MODE == RECT) {
-                    out[(size_t)(a0 + a) * nm2 + b0 + b] = acc[a * MB + b];
-                } else if (!diag || a <= b) {
-                    out[(size_t)(a0 + a) * nm1 + b0 + b] = acc[a * MB + b];
-                    out[(size_t)(b0 + b) * nm1 + a0 + a] = acc[a * MB + b];
-                }
-            }
+#pragma unroll
+    for (int k = 0; k < MB / 4; ++k) {
+        const int a = oa0 + k, b = ob;
+        if (a >= na || b >= nb) continue;
+        if (MODE == SYMM_RFP) {
+            if (!diag || a <= b) out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = acc[k];
+        } else if (MODE == RECT) {
+            out[(size_t)(a0 + a) * nm2 + b0 + b] = acc[k];
+        } else if (!diag || a <= b) {
+            out[(size_t)(a0 + a) * nm1 + b0 + b] = acc[k];
+            out[(size_t)(b0 + b) * nm1 + a0 + a] = acc[k];
         }
     }
 }

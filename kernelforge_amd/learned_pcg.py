@@ -74,6 +74,10 @@ class DeviceSystem:
     def free_rfp(self):
         self.arf = None
 
+    def drop_jacobians(self):
+        """Release dX (only the matrix-free operator and kernel builds need it)."""
+        self.dX = None
+
     def features(self, lam):
         """Per-conformer network input (mean descriptor, log sigma, log10 lambda)."""
         X = self.X.numpy()
