@@ -347,6 +347,46 @@ def learned_pcg_leg(args, L):
     }
 
 
+def traffic_probe(args, L):
+    """Dispatches for the HBM-traffic counters (run under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE): a 1 GiB device-to-device copy, whose counters calibrate the scale, then
+    one FP64 symmetric RFP kernel launch on the n-train energy-fit molecules."""
+    from kernelforge_amd import capi, synthetic
+
+    DA = capi.DeviceArray
+    a, b = DA(1 << 27), DA(1 << 27)
+    a.zero()
+    capi.synchronize()
+    capi.check(L.kf_memcpy(b.ptr, a.ptr, a.nbytes, 2, None), "kf_memcpy")
+    capi.synchronize()
+    log(f"traffic probe: copied {a.nbytes} bytes")
+    del a, b
+    nm = args.n_train
+    coords, zs, _ = synthetic.random_molecules(nm, 1)
+    C, Q, N = synthetic.pad_batch(coords, zs)
+    ma, rs = Q.shape[1], capi.rep_size()
+    dC, dQ, dN = DA.from_numpy(C), DA.from_numpy(Q), DA.from_numpy(N)
+    X = DA((nm, ma, rs))
+    capi.fchl19(dC.ptr, dQ.ptr, dN.ptr, nm, ma, X.ptr)
+    arf = DA(nm * (nm + 1) // 2)
+    capi.synchronize()
+    t0 = time.perf_counter()
+    capi.local_kernel_symm_rfp(capi.KF_PREC_F64, X.ptr, dQ.ptr, dN.ptr, nm, ma, rs, SIGMA, arf.ptr)
+    capi.synchronize()
+    log(f"traffic probe: FP64 symmetric RFP kernel, {nm} molecules, "
+        f"{time.perf_counter() - t0:.3f} s")
+    return 0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            return float(json.load(fh)[kernel]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,7 +400,8 @@ def main(argv=None):
                     help="conformers of the energy+force fit leg (0 skips it)")
     ap.add_argument("--ef-cpu-sample", type=int, default=12,
                     help="conformers in the energy+force CPU-baseline sample (0 skips it)")
-    ap.add_argument("--workload", choices=["energy", "learned-pcg"], default="energy",
+    ap.add_argument("--workload", choices=["energy", "learned-pcg", "traffic-probe"],
+                    default="energy",
                     help="energy: the 20k energy fit (+ the 500-conformer E+F fit); "
                          "learned-pcg: the 2,000-conformer learned-preconditioner E+F fit")
     ap.add_argument("--pcg-conformers", type=int, default=2000)
@@ -382,6 +423,8 @@ def main(argv=None):
     L = capi.lib()
     name = ctypes_name(L)
     log(f"device {name}")
+    if args.workload == "traffic-probe":
+        return traffic_probe(args, L)
     if args.workload == "learned-pcg":
         out = learned_pcg_leg(args, L)
         out["device"] = name
@@ -551,7 +594,7 @@ def main(argv=None):
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": measured_traffic(kname) if head == "fp64" else None,
             "kernel": kname,
             "flop_per_launch": roof_flop,
             "avg_launch_ms": round(avg_ms["kernel"], 3),

@@ -19,7 +19,8 @@
 //    Hessian forms; s_i = sum_j c_m d_m and t_j = sum_i c_m d_m for the E-only
 //    Jacobian forms).  Writes EE / EF / FE, and e_m, v_m, u_m to a per-pair slot.
 //  * deriv_ff_kernel (FP64 MFMA 16x16x4): sum_i J_ai^T T_i with (feature, atom)
-//    chunks of J_a and J_b staged in LDS and T built on the fly as the B operand,
+//    chunks of J_a and J_b staged in LDS and the matching chunk of T formed in LDS
+//    from J_b (wave-uniform atom lists, lanes over columns) before the MFMA loop,
 //    then the rank-1 sum as a second MFMA pass over the slot; writes FF.
 // Every output entry belongs to exactly one molecule pair, so no atomics are used
 // and results are deterministic.
@@ -38,10 +39,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int NT = 256;            // threads per work-group (4 waves)
 constexpr int MAXA = 32;           // atoms per molecule supported here (3*MAXA <= 96 columns)
 constexpr int MAXM = MAXA * MAXA;  // matched atom pairs per molecule pair
-constexpr int VCH = 8;             // difference vectors per projection pass
+constexpr int VCH = 12;            // difference vectors per projection pass
 constexpr int FCH = 128;           // features per staged projection chunk
 constexpr int MAXPB = 6;           // 16-column blocks per side (96 / 16)
-constexpr size_t FF_LDS_BYTES = 64 * 1024;  // J chunks; two work-groups fit one CU
+constexpr size_t FF_LDS_BYTES = 40 * 1024;  // J / T chunks; three work-groups fit one CU
 
 enum DerivMode {
     M_JAC = 0,         // (nm1, naq2)        EF
@@ -82,6 +83,11 @@ struct DerivArgs {
 };
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// LDS row pitch (doubles) for n16 live columns.  Padding the pitch to 16 (mod 32) removes
+// the operand-read bank conflicts but costs a resident work-group per CU, which measured
+// slower (the kernel is latency-bound): rows stay unpadded.
+__host__ __device__ inline int lds_pitch(int n16) { return n16; }
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -198,7 +204,7 @@ __device__ void build_lists(PairLists &L, const int *q1, const int *q2, int a, i
 // cross-slice reduction.  J rows have stride ldj; ncol columns are live.
 template <int NV, class MakeVec, class Emit>
 __device__ void project(const double *J, int ldj, int ncol, int rep, int nv, MakeVec mk,
-                        Emit emit, double (*dv)[FCH], double (*red)[VCH][64]) {
+                        Emit emit, double (*dv)[VCH], double (*red)[VCH][64]) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int pb = 0; pb * 64 < ncol; ++pb) {
         const int p = pb * 64 + lane;
@@ -210,15 +216,28 @@ __device__ void project(const double *J, int ldj, int ncol, int rep, int nv, Mak
             __syncthreads();
             for (int e = tid; e < nv * FCH; e += NT) {
                 const int v = e / FCH, f = e - v * FCH;
-                dv[v][f] = f < fn ? mk(v, f0 + f) : 0.0;
+                dv[f][v] = f < fn ? mk(v, f0 + f) : 0.0;
             }
             __syncthreads();
             if (p < ncol) {
+                // four independent row loads in flight per thread
                 const double *jp = J + (size_t)f0 * ldj + p;
-                for (int f = wv; f < fn; f += 4) {
+                int f = wv;
+                for (; f + 12 < fn; f += 16) {
+                    const double j0 = jp[(size_t)f * ldj], j1 = jp[(size_t)(f + 4) * ldj],
+                                 j2 = jp[(size_t)(f + 8) * ldj], j3 = jp[(size_t)(f + 12) * ldj];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        double t = fma(j0, dv[f][v], acc[v]);
+                        t = fma(j1, dv[f + 4][v], t);
+                        t = fma(j2, dv[f + 8][v], t);
+                        acc[v] = fma(j3, dv[f + 12][v], t);
+                    }
+                }
+                for (; f < fn; f += 4) {
                     const double jv = jp[(size_t)f * ldj];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acc[v] = fma(jv, dv[v][f], acc[v]);
+                    for (int v = 0; v < NV; ++v) acc[v] = fma(jv, dv[f][v], acc[v]);
                 }
             }
         }
@@ -237,7 +256,7 @@ template <int MODE>
 __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) {
     __shared__ PairLists L;
     __shared__ double em[MAXM];
-    __shared__ double dv[VCH][FCH];
+    __shared__ __align__(16) double dv[FCH][VCH];  // feature-major: one row per feature
     __shared__ double red[4][VCH][64];
     __shared__ double part[4][2 * 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -381,7 +400,8 @@ __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) 
 
 // ---------------------------------------------------------------------------
 // FF = sum_(f,i) J_a[i,f,:]^T T[(f,i),:]  -  sum_m w_m v_m u_m^T   on FP64 MFMA.
-// LDS: JA[fc][NI][NPp] | JB[fc][nb][NQp] (dynamic), zero padded.
+// LDS: JA[fc][NI][NPp] | JB[fc][nb][NQp] | T[fc][NI][NQp] (dynamic), zero padded; per
+// feature chunk the work-group stages JA, JB, forms T from JB, then multiplies.
 // Wave w owns q-blocks {qb : qb == w mod 4} (all p-blocks) when there are at least
 // 3 q-blocks; with 1 or 2 q-blocks the waves also split the p-blocks.
 // ---------------------------------------------------------------------------
@@ -417,7 +437,7 @@ __global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, in
     for (int m = tid; m < np; m += NT) mc[m] = slot[m] * A.inv_s2;
 
     const int NI = (na + 3) & ~3, npb = (nca + 15) >> 4, nqb = (ncb + 15) >> 4;
-    const int NPp = npb * 16, NQp = nqb * 16;
+    const int NPp = lds_pitch(npb * 16), NQp = lds_pitch(nqb * 16);
     // wave -> (q-blocks, p-block subset)
     int qlist[2] = {-1, -1}, p0 = 0, pstep = 1;
     if (nqb >= 3) {
@@ -434,6 +454,7 @@ __global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, in
     for (int x = 0; x < MAXPB; ++x) acc[x][0] = acc[x][1] = (v4d){0.0, 0.0, 0.0, 0.0};
 
     double *JA = lds, *JB = lds + (size_t)fc * NI * NPp;
+    double *TT = JB + (size_t)fc * nb * NQp;
     const size_t lda = 3 * (size_t)A.ma1, ldb = 3 * (size_t)A.ma2;
     const double *dxa = A.dx1 + (size_t)a * A.ma1 * rep * lda;
     const double *dxb = A.dx2 + (size_t)b * A.ma2 * rep * ldb;
@@ -457,18 +478,40 @@ __global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, in
                 JB[(size_t)row * NQp + q] = (live && q < ncb) ? src[q] : 0.0;
         }
         __syncthreads();
+        // T[(f, i), q] = sum_{j matched to i} c_ij JB[f][j][q]: one row per wave at a time
+        // (the atom list is wave-uniform), lanes over columns (coalesced LDS reads)
+        for (int row = wv; row < fc * NI; row += 4) {
+            const int f = row / NI, i = row - f * NI;
+            const int mb = i < na ? ist[i] : 0, me = i < na ? ist[i + 1] : 0;
+            const double *jb = JB + (size_t)f * nb * NQp;
+            for (int q = lane; q < NQp; q += 64) {
+                // four independent (index, row) load chains in flight
+                double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+                int m = mb;
+                for (; m + 3 < me; m += 4) {
+                    const int o0 = mj[m] * NQp, o1 = mj[m + 1] * NQp, o2 = mj[m + 2] * NQp,
+                              o3 = mj[m + 3] * NQp;
+                    const double c0 = mc[m], c1 = mc[m + 1], c2 = mc[m + 2], c3 = mc[m + 3];
+                    t0 = fma(c0, jb[o0 + q], t0);
+                    t1 = fma(c1, jb[o1 + q], t1);
+                    t2 = fma(c2, jb[o2 + q], t2);
+                    t3 = fma(c3, jb[o3 + q], t3);
+                }
+                for (; m < me; ++m) t0 = fma(mc[m], jb[mj[m] * NQp + q], t0);
+                TT[(size_t)row * NQp + q] = (t0 + t1) + (t2 + t3);
+            }
+        }
+        __syncthreads();
         const int ksteps = fn * NI / 4;
         for (int s = 0; s < ksteps; ++s) {
             const int kk = 4 * s, f = kk / NI, i = kk - f * NI + kr;
             const double *arow = JA + ((size_t)f * NI + i) * NPp + col;
-            const double *brow = JB + (size_t)f * nb * NQp + col;
-            const int mb = i < na ? ist[i] : 0, me = i < na ? ist[i + 1] : 0;
+            const double *trow = TT + ((size_t)f * NI + i) * NQp + col;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int qb = qlist[k];
                 if (qb < 0) continue;
-                double bv = 0.0;
-                for (int m = mb; m < me; ++m) bv = fma(mc[m], brow[mj[m] * NQp + qb * 16], bv);
+                const double bv = trow[qb * 16];
 #pragma unroll
                 for (int x = 0; x < MAXPB; ++x) {
                     const int pb = p0 + x * pstep;
@@ -596,7 +639,8 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
             workspace(WS_AUX2, batch * A.slot_len * sizeof(double), stream));
         KF_REQUIRE(A.slot, KF_ERR_NOMEM, "scratch allocation failed");
         const int NI = (A.ma1 + 3) & ~3;
-        const int per_f = NI * ((3 * A.ma1 + 15) / 16 * 16) + A.ma2 * ((3 * A.ma2 + 15) / 16 * 16);
+        const int pp = lds_pitch((3 * A.ma1 + 15) / 16 * 16), pq = lds_pitch((3 * A.ma2 + 15) / 16 * 16);
+        const int per_f = NI * pp + A.ma2 * pq + NI * pq;
         fc = std::max(1, std::min(8, int(FF_LDS_BYTES / sizeof(double) / per_f)));
         fc = std::min(fc, A.rep);
         // keep fc * NI a multiple of 4 k-steps: NI already is
@@ -604,8 +648,8 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
         batch = std::min<size_t>(npairs, size_t(1) << 20);
     }
     const int NI = (A.ma1 + 3) & ~3;
-    const size_t lds_ff = sizeof(double) * (size_t)fc *
-                          (NI * ((3 * A.ma1 + 15) / 16 * 16) + A.ma2 * ((3 * A.ma2 + 15) / 16 * 16));
+    const int pp = lds_pitch((3 * A.ma1 + 15) / 16 * 16), pq = lds_pitch((3 * A.ma2 + 15) / 16 * 16);
+    const size_t lds_ff = sizeof(double) * (size_t)fc * (NI * pp + A.ma2 * pq + NI * pq);
     for (size_t p0 = 0; p0 < npairs; p0 += batch) {
         const unsigned nb = (unsigned)std::min(batch, npairs - p0);
         hipLaunchKernelGGL(deriv_prep_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0);

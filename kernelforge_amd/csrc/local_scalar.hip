@@ -20,7 +20,9 @@
 //    atomics; each K entry is written exactly once, straight into RFP (UPLO='U') or
 //    dense layout.  64-molecule blocks keep the 64-row label sub-tiles ~81% full on
 //    ragged molecules (32-molecule blocks: ~67%).
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "local_mixed.hpp"
 
@@ -60,16 +62,6 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
     if (lane == 0) n2[row] = acc;
 }
 
-__device__ inline void decode_upper(int t, int nb, int &A, int &B) {
-    // t enumerates (A, B), A <= B, row by row: offset(A) = A*nb - A(A-1)/2
-    double dn = 2.0 * nb + 1.0;
-    int a = (int)floor((dn - sqrt(dn * dn - 8.0 * t)) * 0.5);
-    if (a < 0) a = 0;
-    while (a > 0 && (long)a * nb - (long)a * (a - 1) / 2 > t) --a;
-    while ((long)(a + 1) * nb - (long)(a + 1) * a / 2 <= t) ++a;
-    A = a;
-    B = a + (int)(t - ((long)a * nb - (long)a * (a - 1) / 2));
-}
 
 // Output block: 64 x 64 molecule pairs per work-group.  The block's K entries live in
 // registers: thread t owns column molecule b = t % 64 and row molecules
@@ -82,7 +74,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
     const double *__restrict__ Xp2, const double *__restrict__ n2b, const int *__restrict__ mr2,
     int L, int nm1, int nm2, int nblk1, int nblk2, int rep_pad, double cexp,
-    double *__restrict__ out) {
+    const int2 *__restrict__ order, double *__restrict__ out) {
     __shared__ double As[TS * LDA];
     __shared__ double Bs[TS * LDA];
     __shared__ double Es[TS * EP];
@@ -91,13 +83,15 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    int A, B;
-    if (MODE == RECT) {
-        A = blockIdx.x / nblk2;
-        B = blockIdx.x % nblk2;
-    } else {
-        decode_upper(blockIdx.x, nblk1, A, B);
-    }
+    // work-group -> output block through the host-built order (8 x 8 super-blocks of
+    // blocks); the hardware deals work-groups round-robin over the 8 XCDs, so logical
+    // slot (g % 8) * (G / 8) + g / 8 gives each XCD a contiguous run of the order and
+    // its L2 sees the same row and column operands repeatedly.
+    const int g = blockIdx.x, G = gridDim.x;
+    const int2 ab = order[(G % 8 == 0) ? (g % 8) * (G / 8) + g / 8 : g];
+    const int A = ab.x, B = ab.y;
+    (void)nblk1;
+    (void)nblk2;
     const int a0 = A * MB, na = min(MB, nm1 - a0);
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
     const bool diag = (MODE != RECT) && (A == B);
@@ -147,28 +141,33 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
 
                 // ---- Gram sub-tile on MFMA ----
                 v4d c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
-                const int lrow = tid >> 2, lk = (tid & 3) * 4;
+                // loader: thread -> (row lrow, KC/4 consecutive k), as double2 pairs
+                constexpr int LV = KC / 8;
+                const int lrow = tid >> 2, lk = (tid & 3) * (KC / 4);
                 const double *pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + lk;
                 const double *pb = Xp2 + (size_t)(c0 + lrow) * rep_pad + lk;
                 const bool va = lrow < rn, vb = lrow < cn;
-                double2 ra_lo = {0, 0}, ra_hi = {0, 0}, rb_lo = {0, 0}, rb_hi = {0, 0};
-                if (va) { ra_lo = *(const double2 *)pa; ra_hi = *(const double2 *)(pa + 2); }
-                if (vb) { rb_lo = *(const double2 *)pb; rb_hi = *(const double2 *)(pb + 2); }
+                double2 ra[LV], rb[LV];
+#pragma unroll
+                for (int u = 0; u < LV; ++u) {
+                    ra[u] = va ? *(const double2 *)(pa + 2 * u) : double2{0, 0};
+                    rb[u] = vb ? *(const double2 *)(pb + 2 * u) : double2{0, 0};
+                }
                 for (int k0 = 0; k0 < rep_pad; k0 += KC) {
                     __syncthreads();
                     double *as = As + lrow * LDA + lk;
                     double *bs = Bs + lrow * LDA + lk;
-                    as[0] = ra_lo.x; as[1] = ra_lo.y; as[2] = ra_hi.x; as[3] = ra_hi.y;
-                    bs[0] = rb_lo.x; bs[1] = rb_lo.y; bs[2] = rb_hi.x; bs[3] = rb_hi.y;
+#pragma unroll
+                    for (int u = 0; u < LV; ++u) {
+                        as[2 * u] = ra[u].x; as[2 * u + 1] = ra[u].y;
+                        bs[2 * u] = rb[u].x; bs[2 * u + 1] = rb[u].y;
+                    }
                     __syncthreads();
                     if (k0 + KC < rep_pad) {  // prefetch the next chunk
-                        if (va) {
-                            ra_lo = *(const double2 *)(pa + k0 + KC);
-                            ra_hi = *(const double2 *)(pa + k0 + KC + 2);
-                        }
-                        if (vb) {
-                            rb_lo = *(const double2 *)(pb + k0 + KC);
-                            rb_hi = *(const double2 *)(pb + k0 + KC + 2);
+#pragma unroll
+                        for (int u = 0; u < LV; ++u) {
+                            if (va) ra[u] = *(const double2 *)(pa + k0 + KC + 2 * u);
+                            if (vb) rb[u] = *(const double2 *)(pb + k0 + KC + 2 * u);
                         }
                     }
 #pragma unroll
@@ -291,6 +290,27 @@ int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, i
     return KF_OK;
 }
 
+// Output-block order: 8 x 8 super-blocks of blocks, super-block rows then columns,
+// blocks inside a super-block row-major (upper triangle only when symmetric).
+int upload_block_order(int nb1, int nb2, bool symm, hipStream_t stream, const int2 **dev) {
+    constexpr int S = 8;
+    std::vector<int2> order;
+    order.reserve(symm ? (size_t)nb1 * (nb1 + 1) / 2 : (size_t)nb1 * nb2);
+    for (int P = 0; P < nb1; P += S)
+        for (int Q = symm ? P : 0; Q < nb2; Q += S)
+            for (int a = P; a < std::min(P + S, nb1); ++a)
+                for (int b = Q; b < std::min(Q + S, nb2); ++b)
+                    if (!symm || a <= b) order.push_back(make_int2(a, b));
+    const size_t bytes = order.size() * sizeof(int2);
+    int2 *d = static_cast<int2 *>(workspace(WS_AUX1, bytes, stream));
+    int2 *h = static_cast<int2 *>(host_staging(WS_AUX1, bytes));
+    KF_REQUIRE(d && h, KF_ERR_NOMEM, "workspace allocation failed");
+    std::copy(order.begin(), order.end(), h);
+    KF_HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+    *dev = d;
+    return KF_OK;
+}
+
 int check_common(const double *X, const int *Q, const int *N, int nm, int ma, int rep,
                  double sigma) {
     KF_REQUIRE(nm >= 0 && ma >= 0 && rep >= 1, KF_ERR_ARG, "bad dimensions");
@@ -324,10 +344,12 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
         KF_HIP_TRY(hipMemsetAsync(out, 0, n * sizeof(double), stream));
         return KF_OK;
     }
+    const int2 *order = nullptr;
+    if ((rc = upload_block_order(nblk, nblk, true, stream, &order))) return rc;
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3((unsigned)ntile), dim3(NT), 0, stream,
                        d.Xp, d.n2, d.mol_row, d.Xp, d.n2, d.mol_row, p.nlabels, nm, nm, nblk,
-                       nblk, d.rep_pad, cexp, static_cast<double *>(out));
+                       nblk, d.rep_pad, cexp, order, static_cast<double *>(out));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
@@ -385,10 +407,12 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     if ((rc = upload_and_pack(p2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
     const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
     const double cexp = -1.0 / (2.0 * sigma * sigma);
+    const int2 *order = nullptr;
+    if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order))) return rc;
     ScopedTimer timer(TM_LOCAL_RECT, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3((unsigned)((long)nblk1 * nblk2)),
                        dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d2.Xp, d2.n2, d2.mol_row,
-                       (int)labels.size(), nm1, nm2, nblk1, nblk2, d1.rep_pad, cexp,
+                       (int)labels.size(), nm1, nm2, nblk1, nblk2, d1.rep_pad, cexp, order,
                        static_cast<double *>(K));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;

@@ -103,24 +103,82 @@ __global__ void add_diag(T *a, int n, double l2) {
     if (j < n) a[rfp_lower((size_t)n, j, j)] += T(l2);
 }
 
-// Inverse of a lower-triangular w x w LDS tile L (pitch TP) into Li (zeroed by the
-// caller), 256 threads: rows advance in sequence, the 64 columns of a row in
-// parallel, each dot product split over 4 threads.
-template <typename T, typename U>
-__device__ inline void tri_inverse(const T *L, U *Li, U *red, int w) {
-    const int c = threadIdx.x & 63, part = threadIdx.x >> 6;
-    for (int r = 0; r < w; ++r) {
-        U s = U(0);
-        if (c < r)
-            for (int q = c + part; q < r; q += 4) s += (U)L[r * TP + q] * Li[q * TP + c];
-        red[part * 64 + c] = s;
-        __syncthreads();
-        if (part == 0 && c <= r) {
-            const U tot = red[c] + red[64 + c] + red[128 + c] + red[192 + c];
-            Li[r * TP + c] = ((r == c ? U(1) : U(0)) - tot) / (U)L[r * TP + r];
+// In-place Cholesky of the lower triangle of a w x w LDS tile L (pitch TP), 256
+// threads on a 16 x 16 grid, one barrier per column.  Phase j reads the unscaled
+// column j and the lower trailing matrix and writes the trailing update
+// L[i][c] -= L[i][j] L[c][j] / L[j][j] (c > j), the scaled column into the unused upper
+// triangle (L[j][i]) and the pivot root into dg[j]; nothing read in a phase is written
+// in it by another thread.  The lower triangle is restored from the upper at the end.
+// Returns 0, or the 1-based column whose pivot was not positive (uniform over threads).
+template <typename T>
+__device__ inline int chol_tile(T *L, T *dg, int w) {
+    const int ti = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    for (int j = 0; j < w; ++j) {
+        const T piv = L[j * TP + j];
+        if (!(piv > T(0))) return j + 1;  // also catches NaN
+        const T d = sqrt(piv), rd = T(1) / d, rp = T(1) / piv;
+        for (int i = j + ti; i < w; i += 16) {
+            const T lij = L[i * TP + j];
+            for (int c = j + tc; c <= i; c += 16) {
+                if (c > j)
+                    L[i * TP + c] -= lij * L[c * TP + j] * rp;
+                else if (i > j)
+                    L[j * TP + i] = lij * rd;
+                else
+                    dg[j] = d;
+            }
         }
         __syncthreads();
     }
+    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+        const int i = idx / w, c = idx - i * w;
+        if (i > c) L[i * TP + c] = L[c * TP + i];
+        else if (i == c) L[i * TP + i] = dg[i];
+    }
+    __syncthreads();
+    return 0;
+}
+
+// Li = L^-1 for a lower-triangular w x w LDS tile (pitch TP), right-looking
+// substitution on the identity, one barrier per row: phase q scales row q into the
+// unused upper triangle of Li (and dgi[q]) and subtracts L[r][q] * row q / L[q][q] from
+// every row r > q; the lower triangle is restored at the end.  Li is w x w, fully
+// written (zeros above the diagonal).
+template <typename T, typename U>
+__device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
+    const int ti = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+        const int i = idx / w, c = idx - i * w;
+        Li[i * TP + c] = (i == c) ? U(1) : U(0);
+    }
+    __syncthreads();
+    for (int q = 0; q < w; ++q) {
+        const U rq = U(1) / (U)L[q * TP + q];
+        if (ti == 0)
+            for (int c = tc; c <= q; c += 16) {
+                const U v = Li[q * TP + c] * rq;
+                if (c < q) Li[c * TP + q] = v; else dgi[q] = v;
+            }
+        for (int r = q + 1 + ti; r < w; r += 16) {
+            const U lr = (U)L[r * TP + q] * rq;
+            for (int c = tc; c <= q; c += 16) Li[r * TP + c] -= lr * Li[q * TP + c];
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+        const int i = idx / w, c = idx - i * w;
+        if (i > c) {
+            Li[i * TP + c] = Li[c * TP + i];
+        } else if (i == c) {
+            Li[i * TP + i] = dgi[i];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+        const int i = idx / w, c = idx - i * w;
+        if (i < c) Li[i * TP + c] = U(0);
+    }
+    __syncthreads();
 }
 
 // POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB].
@@ -128,43 +186,23 @@ template <typename T>
 __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
     __shared__ T L[NB * TP];
     __shared__ T Li[NB * TP];
-    __shared__ int fail;
+    __shared__ T dg[NB];
     if (*info) return;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
     load_tile<T, true>(a, t, w, w, L);
-    if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    for (int j = 0; j < w; ++j) {
-        const T piv = L[j * TP + j];
-        if (!(piv > T(0))) {  // also catches NaN
-            if (threadIdx.x == 0) fail = s0 + j + 1;
-            break;
-        }
-        const T d = sqrt(piv);
-        __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < w; i += blockDim.x) L[i * TP + j] /= d;
-        if (threadIdx.x == 0) L[j * TP + j] = d;
-        __syncthreads();
-        const int m = w - j - 1;
-        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-            const int i = j + 1 + idx / m, c = j + 1 + idx % m;
-            if (c <= i) L[i * TP + c] -= L[i * TP + j] * L[c * TP + j];
-        }
-        __syncthreads();
-    }
-    __syncthreads();
+    const int fail = chol_tile<T>(L, dg, w);
     if (fail) {
-        if (threadIdx.x == 0) atomicCAS(info, 0, fail);
+        if (threadIdx.x == 0) atomicCAS(info, 0, s0 + fail);
         return;
     }
-    // inverse of the lower factor
-    __shared__ T red[256];
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) Li[idx / NB * TP + idx % NB] = T(0);
-    __syncthreads();
-    tri_inverse<T, T>(L, Li, red, w);
+    tri_inverse<T, T>(L, Li, dg, w);
     store_tile<T, true>(a, t, w, w, L);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) inv[idx] = Li[(idx / NB) * TP + idx % NB];
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+        const int r = idx / NB, c = idx % NB;
+        inv[idx] = (r < w && c < w) ? Li[r * TP + c] : T(0);
+    }
 }
 
 // Panel: A_IJ <- A_IJ * inv^T for every block row I > J (one WG per I).
@@ -339,12 +377,14 @@ __global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *
         L[r * TP + c] = v;
         Li[r * TP + c] = 0.0;
     }
-    __shared__ double red[256];
+    __shared__ double dgi[NB];
     __syncthreads();
-    tri_inverse<double, double>(L, Li, red, w);
+    tri_inverse<double, double>(L, Li, dgi, w);
     double *dst = inv + (size_t)J * NB * NB;
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x)
-        dst[idx] = Li[(idx / NB) * TP + idx % NB];
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+        const int r = idx / NB, c = idx % NB;
+        dst[idx] = (r < w && c < w) ? Li[r * TP + c] : 0.0;
+    }
 }
 
 // forward step J: y_J = Linv_J b_J (stored to y by WG 0); b_I -= L_IJ y_J for I > J.
