@@ -65,10 +65,12 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
 
 // Output block: 64 x 64 molecule pairs per work-group.  The block's K entries live in
 // registers: thread t owns column molecule b = t % 64 and row molecules
-// 16 (t / 64) .. +15.  Per sub-tile the exp tile is reduced in place to row-wise run
-// sums over column molecules (a run is the columns of one molecule; run index <= column
-// index, so the in-place write never overtakes the read), then each thread adds the
-// rows of its 16 row molecules for its column molecule's run.
+// 16 (t / 64) .. +15.  Per label, the row sub-tile sr is multiplied with two column
+// sub-tiles at once (the row operand is staged once for both, 3 operand panels per 2
+// output tiles instead of 4), then each tile's exp tile is reduced in place to row-wise
+// run sums over column molecules (a run is the columns of one molecule; run index <=
+// column index, so the in-place write never overtakes the read), and each thread adds
+// the rows of its 16 row molecules for its column molecule's run.
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
@@ -76,17 +78,16 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     int L, int nm1, int nm2, int nblk1, int nblk2, int rep_pad, double cexp,
     const int2 *__restrict__ order, double *__restrict__ out) {
     __shared__ double As[TS * LDA];
-    __shared__ double Bs[TS * LDA];
+    __shared__ double Bs[2][TS * LDA];
     __shared__ double Es[TS * EP];
-    __shared__ int rowmol[TS], colmol[TS], colrun[TS], run_of_mol[MB];
+    __shared__ int rowmol[TS], colmol[2][TS], colrun[2][TS], run_of_mol[2][MB];
     __shared__ int mrA[MB + 1], mrB[MB + 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     // work-group -> output block through the host-built order (8 x 8 super-blocks of
     // blocks); the hardware deals work-groups round-robin over the 8 XCDs, so logical
-    // slot (g % 8) * (G / 8) + g / 8 gives each XCD a contiguous run of the order and
-    // its L2 sees the same row and column operands repeatedly.
+    // slot (g % 8) * (G / 8) + g / 8 gives each XCD a contiguous run of the order.
     const int g = blockIdx.x, G = gridDim.x;
     const int2 ab = order[(G % 8 == 0) ? (g % 8) * (G / 8) + g / 8 : g];
     const int A = ab.x, B = ab.y;
@@ -109,11 +110,16 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
         if (ra1 == ra0 || rb1 == rb0) continue;
         const int nsr = (ra1 - ra0 + TS - 1) / TS, nsc = (rb1 - rb0 + TS - 1) / TS;
         for (int sr = 0; sr < nsr; ++sr) {
-            for (int sc = diag ? sr : 0; sc < nsc; ++sc) {
+            for (int sc0 = diag ? sr : 0; sc0 < nsc; sc0 += 2) {
                 const int r0 = ra0 + sr * TS, rn = min(TS, ra1 - r0);
-                const int c0 = rb0 + sc * TS, cn = min(TS, rb1 - c0);
-                __syncthreads();  // the previous sub-tile's stage 2 is done with Es
-                if (tid >= 128 && tid < 128 + MB) run_of_mol[tid - 128] = -1;
+                const int ntile = sc0 + 1 < nsc ? 2 : 1;
+                int c0[2], cn[2];
+                c0[0] = rb0 + sc0 * TS;
+                cn[0] = min(TS, rb1 - c0[0]);
+                c0[1] = c0[0] + TS;
+                cn[1] = ntile == 2 ? min(TS, rb1 - c0[1]) : 0;
+                __syncthreads();  // the previous tiles' stage 2 is done with Es
+                if (tid >= 128) run_of_mol[(tid - 128) >> 6][(tid - 128) & 63] = -1;
                 __syncthreads();
                 if (wave == 0) {  // molecule of every row
                     int m = -1;
@@ -123,51 +129,63 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                         while (mrA[m + 1] <= r) ++m;
                     }
                     rowmol[lane] = m;
-                } else if (wave == 1) {  // molecule and run of every column
+                } else if (wave <= 2) {  // molecule and run of every column of tile wave-1
+                    const int u = wave - 1;
                     int m = -1;
-                    if (lane < cn) {
-                        const int r = c0 + lane;
+                    if (lane < cn[u]) {
+                        const int r = c0[u] + lane;
                         m = 0;
                         while (mrB[m + 1] <= r) ++m;
                     }
                     const int prev = __shfl_up(m, 1, 64);
-                    const bool head = lane < cn && (lane == 0 || prev != m);
+                    const bool head = lane < cn[u] && (lane == 0 || prev != m);
                     const unsigned long long heads = __ballot(head);
                     const int run = __popcll(heads & ((2ull << lane) - 1)) - 1;
-                    colmol[lane] = m;
-                    colrun[lane] = run;
-                    if (head) run_of_mol[m] = run;
+                    colmol[u][lane] = m;
+                    colrun[u][lane] = run;
+                    if (head) run_of_mol[u][m] = run;
                 }
 
-                // ---- Gram sub-tile on MFMA ----
-                v4d c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+                // ---- two Gram sub-tiles on MFMA, the row operand staged once ----
+                v4d c[2][2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int x = 0; x < 2; ++x)
+#pragma unroll
+                        for (int y = 0; y < 2; ++y) c[u][x][y] = (v4d){0, 0, 0, 0};
                 // loader: thread -> (row lrow, KC/4 consecutive k), as double2 pairs
                 constexpr int LV = KC / 8;
                 const int lrow = tid >> 2, lk = (tid & 3) * (KC / 4);
                 const double *pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + lk;
-                const double *pb = Xp2 + (size_t)(c0 + lrow) * rep_pad + lk;
-                const bool va = lrow < rn, vb = lrow < cn;
-                double2 ra[LV], rb[LV];
+                const double *pb0 = Xp2 + (size_t)(c0[0] + lrow) * rep_pad + lk;
+                const double *pb1 = Xp2 + (size_t)(c0[1] + lrow) * rep_pad + lk;
+                const bool va = lrow < rn, vb0 = lrow < cn[0], vb1 = lrow < cn[1];
+                double2 ra[LV], rb0[LV], rb1[LV];
 #pragma unroll
                 for (int u = 0; u < LV; ++u) {
                     ra[u] = va ? *(const double2 *)(pa + 2 * u) : double2{0, 0};
-                    rb[u] = vb ? *(const double2 *)(pb + 2 * u) : double2{0, 0};
+                    rb0[u] = vb0 ? *(const double2 *)(pb0 + 2 * u) : double2{0, 0};
+                    rb1[u] = vb1 ? *(const double2 *)(pb1 + 2 * u) : double2{0, 0};
                 }
                 for (int k0 = 0; k0 < rep_pad; k0 += KC) {
                     __syncthreads();
                     double *as = As + lrow * LDA + lk;
-                    double *bs = Bs + lrow * LDA + lk;
+                    double *bs0 = Bs[0] + lrow * LDA + lk;
+                    double *bs1 = Bs[1] + lrow * LDA + lk;
 #pragma unroll
                     for (int u = 0; u < LV; ++u) {
                         as[2 * u] = ra[u].x; as[2 * u + 1] = ra[u].y;
-                        bs[2 * u] = rb[u].x; bs[2 * u + 1] = rb[u].y;
+                        bs0[2 * u] = rb0[u].x; bs0[2 * u + 1] = rb0[u].y;
+                        bs1[2 * u] = rb1[u].x; bs1[2 * u + 1] = rb1[u].y;
                     }
                     __syncthreads();
                     if (k0 + KC < rep_pad) {  // prefetch the next chunk
 #pragma unroll
                         for (int u = 0; u < LV; ++u) {
                             if (va) ra[u] = *(const double2 *)(pa + k0 + KC + 2 * u);
-                            if (vb) rb[u] = *(const double2 *)(pb + k0 + KC + 2 * u);
+                            if (vb0) rb0[u] = *(const double2 *)(pb0 + k0 + KC + 2 * u);
+                            if (vb1) rb1[u] = *(const double2 *)(pb1 + k0 + KC + 2 * u);
                         }
                     }
 #pragma unroll
@@ -175,64 +193,74 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                         const int kk = ks * 4 + (lane >> 4);
                         const double a_0 = As[(wr * 32 + (lane & 15)) * LDA + kk];
                         const double a_1 = As[(wr * 32 + 16 + (lane & 15)) * LDA + kk];
-                        const double b_0 = Bs[(wc * 32 + (lane & 15)) * LDA + kk];
-                        const double b_1 = Bs[(wc * 32 + 16 + (lane & 15)) * LDA + kk];
-                        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_0, c00, 0, 0, 0);
-                        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_1, c01, 0, 0, 0);
-                        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_0, c10, 0, 0, 0);
-                        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_1, c11, 0, 0, 0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            if (u == 1 && ntile == 1) break;
+                            const double b_0 = Bs[u][(wc * 32 + (lane & 15)) * LDA + kk];
+                            const double b_1 = Bs[u][(wc * 32 + 16 + (lane & 15)) * LDA + kk];
+                            c[u][0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_0, c[u][0][0], 0, 0, 0);
+                            c[u][0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_1, c[u][0][1], 0, 0, 0);
+                            c[u][1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_0, c[u][1][0], 0, 0, 0);
+                            c[u][1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_1, c[u][1][1], 0, 0, 0);
+                        }
                     }
                 }
-                // ---- epilogue: exp into LDS ----
-                const bool dbl_self = diag && sc > sr;  // mirror sub-tile not visited
+                // ---- per tile: exp into LDS, in-place row runs, register accumulation ----
 #pragma unroll
-                for (int tm = 0; tm < 2; ++tm) {
+                for (int u = 0; u < 2; ++u) {
+                    if (u == 1 && ntile == 1) break;
+                    const int cb = c0[u], cnu = cn[u];
+                    const bool dbl_self = diag && sc0 + u > sr;  // mirror sub-tile not visited
+                    __syncthreads();  // Es free (tile 0's stage 2 is done before tile 1)
 #pragma unroll
-                    for (int tn = 0; tn < 2; ++tn) {
-                        const v4d cv = tm == 0 ? (tn == 0 ? c00 : c01) : (tn == 0 ? c10 : c11);
-                        const int col = wc * 32 + tn * 16 + (lane & 15);
+                    for (int tm = 0; tm < 2; ++tm) {
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const int row = wr * 32 + tm * 16 + mfma16_row_f64(lane, v);
-                            double e = 0.0;
-                            if (row < rn && col < cn) {
-                                const double d2 = n2a[r0 + row] + n2b[c0 + col] - 2.0 * cv[v];
-                                e = exp(d2 * cexp);
-                                if (dbl_self && rowmol[row] == colmol[col]) e *= 2.0;
+                        for (int tn = 0; tn < 2; ++tn) {
+                            const v4d cv = c[u][tm][tn];
+                            const int col = wc * 32 + tn * 16 + (lane & 15);
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int row = wr * 32 + tm * 16 + mfma16_row_f64(lane, v);
+                                double e = 0.0;
+                                if (row < rn && col < cnu) {
+                                    const double d2 = n2a[r0 + row] + n2b[cb + col] - 2.0 * cv[v];
+                                    e = exp(d2 * cexp);
+                                    if (dbl_self && rowmol[row] == colmol[u][col]) e *= 2.0;
+                                }
+                                Es[row * EP + col] = e;
                             }
-                            Es[row * EP + col] = e;
                         }
                     }
-                }
-                __syncthreads();
-                // stage 1: row-wise run sums over column molecules, in place
-                if (tid < rn) {
-                    double *er = Es + tid * EP;
-                    int cur = 0;
-                    double s = 0.0;
-                    for (int j = 0; j < cn; ++j) {
-                        const int rj = colrun[j];
-                        if (rj != cur) {
-                            er[cur] = s;
-                            s = 0.0;
-                            cur = rj;
+                    __syncthreads();
+                    // stage 1: row-wise run sums over column molecules, in place
+                    if (tid < rn) {
+                        double *er = Es + tid * EP;
+                        int cur = 0;
+                        double sum = 0.0;
+                        for (int j = 0; j < cnu; ++j) {
+                            const int rj = colrun[u][j];
+                            if (rj != cur) {
+                                er[cur] = sum;
+                                sum = 0.0;
+                                cur = rj;
+                            }
+                            sum += er[j];
                         }
-                        s += er[j];
+                        er[cur] = sum;
                     }
-                    er[cur] = s;
-                }
-                __syncthreads();
-                // stage 2: each thread adds the rows of its row molecules for its column
-                const int run = ob < nb ? run_of_mol[ob] : -1;
-                if (run >= 0) {
+                    __syncthreads();
+                    // stage 2: each thread adds the rows of its row molecules for its column
+                    const int run = ob < nb ? run_of_mol[u][ob] : -1;
+                    if (run >= 0) {
 #pragma unroll
-                    for (int k = 0; k < MB / 4; ++k) {
-                        const int a = oa0 + k;
-                        if (a < na) {
-                            const int rs = max(mrA[a], r0) - r0, re = min(mrA[a + 1], r0 + rn) - r0;
-                            double s = 0.0;
-                            for (int i = rs; i < re; ++i) s += Es[i * EP + run];
-                            acc[k] += s;
+                        for (int k = 0; k < MB / 4; ++k) {
+                            const int a = oa0 + k;
+                            if (a < na) {
+                                const int rs = max(mrA[a], r0) - r0, re = min(mrA[a + 1], r0 + rn) - r0;
+                                double sum = 0.0;
+                                for (int i = rs; i < re; ++i) sum += Es[i * EP + run];
+                                acc[k] += sum;
+                            }
                         }
                     }
                 }
