@@ -63,40 +63,157 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
 }
 
 
+// One k-chunk (KC) of the wave's Gram fragments: NA 16-row fragments of the row tile
+// times the wave's 16 columns of NTW column tiles, as v_mfma_f64_16x16x4 (A[i][k] from
+// lane i + 16k, B[k][j] from lane j + 16k, D row 4v + lane/16, column lane % 16).
+// The operands of k-step ks+1 are read from LDS while the MFMAs of ks issue.
+template <int NA, int NTW>
+__device__ __forceinline__ void mma_chunk(const double *__restrict__ As,
+                                          const double *__restrict__ B0,
+                                          const double *__restrict__ B1, v4d (&c)[2][4],
+                                          int r16, int bcol, int kq) {
+    if (NTW == 0) return;
+    double a[2][NA], b[2][NTW];
+    auto read = [&](int ks, int s) {
+        const int kk = ks * 4 + kq;
+#pragma unroll
+        for (int x = 0; x < NA; ++x) a[s][x] = As[(x * 16 + r16) * LDA + kk];
+        b[s][0] = B0[bcol * LDA + kk];
+        if (NTW == 2) b[s][NTW - 1] = B1[bcol * LDA + kk];
+    };
+    read(0, 0);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+        if (ks + 1 < KC / 4) read(ks + 1, (ks + 1) & 1);
+#pragma unroll
+        for (int u = 0; u < NTW; ++u)
+#pragma unroll
+            for (int x = 0; x < NA; ++x)
+                c[u][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks & 1][x], b[ks & 1][u],
+                                                               c[u][x], 0, 0, 0);
+    }
+}
+
+// Operand pointers and predicates of one row tile and NTL column tiles.
+struct GramArgs {
+    const double *pa, *pb0, *pb1;
+    bool va, vb0, vb1;
+    int rep_pad, lofs, r16, bcol, kq;
+};
+
+// Gram products of one row sub-tile with NTL column sub-tiles over the full depth.
+// Operands go global -> registers -> LDS with two LDS buffers: while the MFMAs run on
+// chunk k, chunk k+1 is stored into the other buffer and chunk k+2 is loaded into
+// registers, so each chunk costs one barrier.  Every wave stages its share of all NTL
+// tiles but multiplies only its NA x NTW non-empty fragments.  On entry the caller's
+// last barrier has freed both buffers; on exit a barrier has retired every read.
+template <int NTL, int NA, int NTW>
+__device__ void gram_tiles(double *__restrict__ Ops, const GramArgs &g, v4d (&c)[2][4]) {
+    constexpr int LV = KC / 8, BUF = 3 * TS * LDA;
+    double2 ra[LV], rb0[LV], rb1[LV];
+    auto load = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < LV; ++u) {
+            ra[u] = g.va ? *(const double2 *)(g.pa + k + 2 * u) : double2{0, 0};
+            rb0[u] = g.vb0 ? *(const double2 *)(g.pb0 + k + 2 * u) : double2{0, 0};
+            if (NTL == 2) rb1[u] = g.vb1 ? *(const double2 *)(g.pb1 + k + 2 * u) : double2{0, 0};
+        }
+    };
+    auto store = [&](double *buf) {
+        double *as = buf + g.lofs, *bs0 = buf + TS * LDA + g.lofs;
+        double *bs1 = buf + 2 * TS * LDA + g.lofs;
+#pragma unroll
+        for (int u = 0; u < LV; ++u) {
+            as[2 * u] = ra[u].x;
+            as[2 * u + 1] = ra[u].y;
+            bs0[2 * u] = rb0[u].x;
+            bs0[2 * u + 1] = rb0[u].y;
+            if (NTL == 2) {
+                bs1[2 * u] = rb1[u].x;
+                bs1[2 * u + 1] = rb1[u].y;
+            }
+        }
+    };
+    const int nch = g.rep_pad / KC;
+    load(0);
+    store(Ops);
+    if (nch > 1) load(KC);
+    __syncthreads();
+    for (int kc = 0; kc < nch; ++kc) {
+        double *cur = Ops + (kc & 1) * BUF;
+        if (kc + 1 < nch) {
+            store(Ops + ((kc + 1) & 1) * BUF);
+            if (kc + 2 < nch) load((kc + 2) * KC);
+        }
+        mma_chunk<NA, NTW>(cur, cur + TS * LDA, cur + 2 * TS * LDA, c, g.r16, g.bcol, g.kq);
+        __syncthreads();
+    }
+}
+
+template <int NTL, int NA>
+__device__ __forceinline__ void gram_ntw(double *Ops, const GramArgs &g, int ntw,
+                                         v4d (&c)[2][4]) {
+    if (ntw == 0) gram_tiles<NTL, NA, 0>(Ops, g, c);
+    else if (NTL == 1 || ntw == 1) gram_tiles<NTL, NA, 1>(Ops, g, c);
+    else gram_tiles<NTL, NA, NTL>(Ops, g, c);
+}
+
+template <int NTL>
+__device__ __forceinline__ void gram_dispatch(double *Ops, const GramArgs &g, int na16, int ntw,
+                                              v4d (&c)[2][4]) {
+    switch (na16) {
+        case 1: gram_ntw<NTL, 1>(Ops, g, ntw, c); break;
+        case 2: gram_ntw<NTL, 2>(Ops, g, ntw, c); break;
+        case 3: gram_ntw<NTL, 3>(Ops, g, ntw, c); break;
+        default: gram_ntw<NTL, 4>(Ops, g, ntw, c); break;
+    }
+}
+
 // Output block: 64 x 64 molecule pairs per work-group.  The block's K entries live in
 // registers: thread t owns column molecule b = t % 64 and row molecules
 // 16 (t / 64) .. +15.  Per label, the row sub-tile sr is multiplied with two column
 // sub-tiles at once (the row operand is staged once for both, 3 operand panels per 2
-// output tiles instead of 4), then each tile's exp tile is reduced in place to row-wise
-// run sums over column molecules (a run is the columns of one molecule; run index <=
-// column index, so the in-place write never overtakes the read), and each thread adds
-// the rows of its 16 row molecules for its column molecule's run.
+// output tiles instead of 4); wave w computes columns 16w .. 16w+15 of both tiles for
+// every non-empty 16-row fragment.  Each tile's exp tile is then reduced to row-wise run
+// sums over column molecules (a run is the columns of one molecule; four threads per
+// row, the partial sums of a run that crosses a 16-column quarter carried across the
+// four lanes), and each thread adds the rows of its 16 row molecules for its column
+// molecule's run.  The molecule of a packed row is src[row] / max_atoms
+// (src = a * max_atoms + i, see local_plan.hpp).
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
-    const double *__restrict__ Xp2, const double *__restrict__ n2b, const int *__restrict__ mr2,
-    int L, int nm1, int nm2, int nblk1, int nblk2, int rep_pad, double cexp,
-    const int2 *__restrict__ order, double *__restrict__ out) {
-    __shared__ double As[TS * LDA];
-    __shared__ double Bs[2][TS * LDA];
-    __shared__ double Es[TS * EP];
-    __shared__ int rowmol[TS], colmol[2][TS], colrun[2][TS], run_of_mol[2][MB];
+    const int *__restrict__ src1, int ma1, const double *__restrict__ Xp2,
+    const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
+    int ma2, int L, int nm1, int nm2, int rep_pad, double cexp, const int2 *__restrict__ order,
+    double *__restrict__ out) {
+    // two operand buffers (row tile + two column tiles each); the exp tile and the run
+    // sums of the epilogue reuse the same memory once the Gram products are done
+    constexpr int OPS = 2 * 3 * TS * LDA, EPI = 2 * TS * EP;
+    __shared__ double Sh[OPS > EPI ? OPS : EPI];
+    double *const Ops = Sh, *const Es = Sh, *const Rs = Sh + TS * EP;
+    __shared__ double n2r[TS], n2c[2][TS];
+    __shared__ int rowmol[TS], colmol[2][TS], colrun[2][TS + 1], run_of_mol[2][MB];
     __shared__ int mrA[MB + 1], mrB[MB + 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
     // work-group -> output block through the host-built order (8 x 8 super-blocks of
     // blocks); the hardware deals work-groups round-robin over the 8 XCDs, so logical
     // slot (g % 8) * (G / 8) + g / 8 gives each XCD a contiguous run of the order.
-    const int g = blockIdx.x, G = gridDim.x;
-    const int2 ab = order[(G % 8 == 0) ? (g % 8) * (G / 8) + g / 8 : g];
+    const int gi = blockIdx.x, G = gridDim.x;
+    const int2 ab = order[(G % 8 == 0) ? (gi % 8) * (G / 8) + gi / 8 : gi];
     const int A = ab.x, B = ab.y;
-    (void)nblk1;
-    (void)nblk2;
     const int a0 = A * MB, na = min(MB, nm1 - a0);
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
     const bool diag = (MODE != RECT) && (A == B);
     const int ob = tid & (MB - 1), oa0 = (tid / MB) * (MB / 4);
+    const int lrow = tid >> 2;
+    GramArgs ga;
+    ga.rep_pad = rep_pad;
+    ga.lofs = lrow * LDA + (tid & 3) * (KC / 4);
+    ga.r16 = lane & 15;
+    ga.bcol = wave * 16 + (lane & 15);
+    ga.kq = lane >> 4;
     double acc[MB / 4];
 #pragma unroll
     for (int k = 0; k < MB / 4; ++k) acc[k] = 0.0;
@@ -118,27 +235,19 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                 cn[0] = min(TS, rb1 - c0[0]);
                 c0[1] = c0[0] + TS;
                 cn[1] = ntile == 2 ? min(TS, rb1 - c0[1]) : 0;
-                __syncthreads();  // the previous tiles' stage 2 is done with Es
-                if (tid >= 128) run_of_mol[(tid - 128) >> 6][(tid - 128) & 63] = -1;
-                __syncthreads();
-                if (wave == 0) {  // molecule of every row
-                    int m = -1;
-                    if (lane < rn) {
-                        const int r = r0 + lane;
-                        m = 0;
-                        while (mrA[m + 1] <= r) ++m;
-                    }
-                    rowmol[lane] = m;
-                } else if (wave <= 2) {  // molecule and run of every column of tile wave-1
+                __syncthreads();  // the previous tiles' epilogue is done with Sh and the maps
+                if (wave == 0) {  // molecule and |x|^2 of every row
+                    const bool v = lane < rn;
+                    rowmol[lane] = v ? src1[r0 + lane] / ma1 - a0 : -1;
+                    n2r[lane] = v ? n2a[r0 + lane] : 0.0;
+                } else if (wave <= 2) {  // molecule, run and |x|^2 of every column of tile wave-1
                     const int u = wave - 1;
-                    int m = -1;
-                    if (lane < cn[u]) {
-                        const int r = c0[u] + lane;
-                        m = 0;
-                        while (mrB[m + 1] <= r) ++m;
-                    }
+                    const bool v = lane < cn[u];
+                    run_of_mol[u][lane] = -1;  // same wave: ordered before the writes below
+                    const int m = v ? src2[c0[u] + lane] / ma2 - b0 : -1;
+                    n2c[u][lane] = v ? n2b[c0[u] + lane] : 0.0;
                     const int prev = __shfl_up(m, 1, 64);
-                    const bool head = lane < cn[u] && (lane == 0 || prev != m);
+                    const bool head = v && (lane == 0 || prev != m);
                     const unsigned long long heads = __ballot(head);
                     const int run = __popcll(heads & ((2ull << lane) - 1)) - 1;
                     colmol[u][lane] = m;
@@ -146,107 +255,90 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     if (head) run_of_mol[u][m] = run;
                 }
 
-                // ---- two Gram sub-tiles on MFMA, the row operand staged once ----
-                v4d c[2][2][2];
+                // ---- Gram sub-tiles on MFMA, the row operand staged once ----
+                v4d c[2][4];
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
-                    for (int x = 0; x < 2; ++x)
-#pragma unroll
-                        for (int y = 0; y < 2; ++y) c[u][x][y] = (v4d){0, 0, 0, 0};
-                // loader: thread -> (row lrow, KC/4 consecutive k), as double2 pairs
-                constexpr int LV = KC / 8;
-                const int lrow = tid >> 2, lk = (tid & 3) * (KC / 4);
-                const double *pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + lk;
-                const double *pb0 = Xp2 + (size_t)(c0[0] + lrow) * rep_pad + lk;
-                const double *pb1 = Xp2 + (size_t)(c0[1] + lrow) * rep_pad + lk;
-                const bool va = lrow < rn, vb0 = lrow < cn[0], vb1 = lrow < cn[1];
-                double2 ra[LV], rb0[LV], rb1[LV];
-#pragma unroll
-                for (int u = 0; u < LV; ++u) {
-                    ra[u] = va ? *(const double2 *)(pa + 2 * u) : double2{0, 0};
-                    rb0[u] = vb0 ? *(const double2 *)(pb0 + 2 * u) : double2{0, 0};
-                    rb1[u] = vb1 ? *(const double2 *)(pb1 + 2 * u) : double2{0, 0};
-                }
-                for (int k0 = 0; k0 < rep_pad; k0 += KC) {
-                    __syncthreads();
-                    double *as = As + lrow * LDA + lk;
-                    double *bs0 = Bs[0] + lrow * LDA + lk;
-                    double *bs1 = Bs[1] + lrow * LDA + lk;
-#pragma unroll
-                    for (int u = 0; u < LV; ++u) {
-                        as[2 * u] = ra[u].x; as[2 * u + 1] = ra[u].y;
-                        bs0[2 * u] = rb0[u].x; bs0[2 * u + 1] = rb0[u].y;
-                        bs1[2 * u] = rb1[u].x; bs1[2 * u + 1] = rb1[u].y;
-                    }
-                    __syncthreads();
-                    if (k0 + KC < rep_pad) {  // prefetch the next chunk
-#pragma unroll
-                        for (int u = 0; u < LV; ++u) {
-                            if (va) ra[u] = *(const double2 *)(pa + k0 + KC + 2 * u);
-                            if (vb0) rb0[u] = *(const double2 *)(pb0 + k0 + KC + 2 * u);
-                            if (vb1) rb1[u] = *(const double2 *)(pb1 + k0 + KC + 2 * u);
-                        }
-                    }
-#pragma unroll
-                    for (int ks = 0; ks < KC / 4; ++ks) {
-                        const int kk = ks * 4 + (lane >> 4);
-                        const double a_0 = As[(wr * 32 + (lane & 15)) * LDA + kk];
-                        const double a_1 = As[(wr * 32 + 16 + (lane & 15)) * LDA + kk];
-#pragma unroll
-                        for (int u = 0; u < 2; ++u) {
-                            if (u == 1 && ntile == 1) break;
-                            const double b_0 = Bs[u][(wc * 32 + (lane & 15)) * LDA + kk];
-                            const double b_1 = Bs[u][(wc * 32 + 16 + (lane & 15)) * LDA + kk];
-                            c[u][0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_0, c[u][0][0], 0, 0, 0);
-                            c[u][0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_0, b_1, c[u][0][1], 0, 0, 0);
-                            c[u][1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_0, c[u][1][0], 0, 0, 0);
-                            c[u][1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_1, b_1, c[u][1][1], 0, 0, 0);
-                        }
-                    }
-                }
-                // ---- per tile: exp into LDS, in-place row runs, register accumulation ----
+                    for (int x = 0; x < 4; ++x) c[u][x] = (v4d){0, 0, 0, 0};
+                ga.pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + (tid & 3) * (KC / 4);
+                ga.pb0 = Xp2 + (size_t)(c0[0] + lrow) * rep_pad + (tid & 3) * (KC / 4);
+                ga.pb1 = ga.pb0 + (size_t)TS * rep_pad;
+                ga.va = lrow < rn;
+                ga.vb0 = lrow < cn[0];
+                ga.vb1 = lrow < cn[1];
+                const int na16 = (rn + 15) >> 4;
+                const int ntw = (wave * 16 < cn[1]) ? 2 : (wave * 16 < cn[0] ? 1 : 0);
+                if (ntile == 2)
+                    gram_dispatch<2>(Ops, ga, na16, ntw, c);
+                else
+                    gram_dispatch<1>(Ops, ga, na16, ntw, c);
+
+                // ---- per tile: exp into LDS, row runs, register accumulation ----
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     if (u == 1 && ntile == 1) break;
-                    const int cb = c0[u], cnu = cn[u];
+                    const int cnu = cn[u];
                     const bool dbl_self = diag && sc0 + u > sr;  // mirror sub-tile not visited
-                    __syncthreads();  // Es free (tile 0's stage 2 is done before tile 1)
+                    if (u == 1) __syncthreads();  // tile 0's stage 2 is done with Es and Rs
+                    {
+                        const int col = wave * 16 + (lane & 15);
+                        const double nc = n2c[u][col];
+                        const int cm = colmol[u][col];
 #pragma unroll
-                    for (int tm = 0; tm < 2; ++tm) {
-#pragma unroll
-                        for (int tn = 0; tn < 2; ++tn) {
-                            const v4d cv = c[u][tm][tn];
-                            const int col = wc * 32 + tn * 16 + (lane & 15);
+                        for (int x = 0; x < 4; ++x) {
+                            const v4d cv = c[u][x];
 #pragma unroll
                             for (int v = 0; v < 4; ++v) {
-                                const int row = wr * 32 + tm * 16 + mfma16_row_f64(lane, v);
+                                const int row = x * 16 + mfma16_row_f64(lane, v);
                                 double e = 0.0;
                                 if (row < rn && col < cnu) {
-                                    const double d2 = n2a[r0 + row] + n2b[cb + col] - 2.0 * cv[v];
-                                    e = exp(d2 * cexp);
-                                    if (dbl_self && rowmol[row] == colmol[u][col]) e *= 2.0;
+                                    e = exp((n2r[row] + nc - 2.0 * cv[v]) * cexp);
+                                    if (dbl_self && rowmol[row] == cm) e *= 2.0;
                                 }
                                 Es[row * EP + col] = e;
                             }
                         }
                     }
                     __syncthreads();
-                    // stage 1: row-wise run sums over column molecules, in place
-                    if (tid < rn) {
-                        double *er = Es + tid * EP;
-                        int cur = 0;
-                        double sum = 0.0;
-                        for (int j = 0; j < cnu; ++j) {
-                            const int rj = colrun[u][j];
-                            if (rj != cur) {
-                                er[cur] = sum;
-                                sum = 0.0;
-                                cur = rj;
+                    // stage 1: run sums of row i over column molecules, four threads per
+                    // row (16 columns each); a run's partial sum is carried to the next
+                    // quarter while the run continues there
+                    {
+                        const int i = tid >> 2, q = tid & 3;
+                        const int *cr = colrun[u];
+                        const double *er = Es + i * EP;
+                        double s = 0.0, first = 0.0;
+                        int rfirst = -1;
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const int col = q * 16 + j;
+                            if (col < cnu) {
+                                s += er[col];
+                                const int rj = cr[col];
+                                if (col == cnu - 1 || cr[col + 1] != rj) {  // run ends here
+                                    if (rfirst < 0) {
+                                        first = s;
+                                        rfirst = rj;
+                                    } else if (i < rn) {
+                                        Rs[i * EP + rj] = s;
+                                    }
+                                    s = 0.0;
+                                }
                             }
-                            sum += er[j];
                         }
-                        er[cur] = sum;
+                        // carry: the open run at the end of quarter q-1 (plus its own
+                        // carry-in when it spans that whole quarter) continues here
+                        double cin = 0.0, cout = s;
+#pragma unroll
+                        for (int step = 1; step < 4; ++step) {
+                            const double prev = __shfl_up(cout, 1, 64);
+                            if (q == step) {
+                                cin = prev;
+                                cout = rfirst < 0 ? cin + s : s;
+                            }
+                        }
+                        if (rfirst >= 0 && i < rn) Rs[i * EP + rfirst] = first + cin;
                     }
                     __syncthreads();
                     // stage 2: each thread adds the rows of its row molecules for its column
@@ -258,7 +350,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                             if (a < na) {
                                 const int rs = max(mrA[a], r0) - r0, re = min(mrA[a + 1], r0 + rn) - r0;
                                 double sum = 0.0;
-                                for (int i = rs; i < re; ++i) sum += Es[i * EP + run];
+                                for (int i = rs; i < re; ++i) sum += Rs[i * EP + run];
                                 acc[k] += sum;
                             }
                         }
@@ -286,7 +378,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
 // Device copy of a plan plus the packed rows.
 struct DevicePacked {
     double *Xp = nullptr, *n2 = nullptr;
-    int *mol_row = nullptr;
+    int *mol_row = nullptr, *src = nullptr;
     int rows = 0, rep_pad = 0;
 };
 
@@ -302,6 +394,7 @@ int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, i
                pack_bytes);
     d.mol_row = reinterpret_cast<int *>(plan);
     int *src = d.mol_row + mr_n;
+    d.src = src;
     int *stage = static_cast<int *>(host_staging(ws_plan, sizeof(int) * (mr_n + src_n)));
     KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
     std::copy(p.mol_row.begin(), p.mol_row.end(), stage);
@@ -376,8 +469,8 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     if ((rc = upload_block_order(nblk, nblk, true, stream, &order))) return rc;
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3((unsigned)ntile), dim3(NT), 0, stream,
-                       d.Xp, d.n2, d.mol_row, d.Xp, d.n2, d.mol_row, p.nlabels, nm, nm, nblk,
-                       nblk, d.rep_pad, cexp, order, static_cast<double *>(out));
+                       d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
+                       p.nlabels, nm, nm, d.rep_pad, cexp, order, static_cast<double *>(out));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
@@ -439,9 +532,9 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order))) return rc;
     ScopedTimer timer(TM_LOCAL_RECT, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3((unsigned)((long)nblk1 * nblk2)),
-                       dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d2.Xp, d2.n2, d2.mol_row,
-                       (int)labels.size(), nm1, nm2, nblk1, nblk2, d1.rep_pad, cexp, order,
-                       static_cast<double *>(K));
+                       dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
+                       d2.n2, d2.mol_row, d2.src, max_atoms2, (int)labels.size(), nm1, nm2,
+                       d1.rep_pad, cexp, order, static_cast<double *>(K));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
