@@ -10,16 +10,18 @@
 //  * Atoms are packed by (label, molecule) once (local_plan.hpp), so the rows of a
 //    molecule block for one label are one contiguous range.
 //  * The OUTPUT is tiled: one workgroup owns a 64x64 block of molecule pairs (upper
-//    triangle of blocks for the symmetric forms).  For every label it runs the
-//    rows(A-block) x rows(B-block) Gram product as 64x64 sub-tiles on FP64 MFMA
-//    (v_mfma_f64_16x16x4, 2x2 waves of 32x32), with operands staged through LDS
-//    and the next k-chunk prefetched into registers during the MFMAs.
-//  * Epilogue fused per sub-tile: squared norms (FP64), exp, then a deterministic
-//    two-stage segmented reduction (rows -> column-molecule runs in LDS, then row
-//    molecules into per-thread register accumulators of the 64x64 block).  No
+//    triangle of blocks for the symmetric forms), accumulated in LDS.  For every label
+//    it runs the rows(A-block) x rows(B-block) Gram product as 64-row sub-tiles on FP64
+//    MFMA (v_mfma_f64_16x16x4): one row tile against two column tiles at a time; wave
+//    w owns columns 16w..16w+15 and skips empty 16-row/16-column fragments (95% of
+//    the MFMAs issued are useful on ragged molecules, 81% with whole 64x64 tiles).  The
+//    shared row tile is staged through double-buffered LDS, the wave's own column
+//    operands stream from global memory straight into registers.
+//  * Epilogue fused per tile, in registers: squared norms (FP64), exp, a segmented
+//    scan along each wave's 16 columns (DPP row shifts) that sums the columns of one
+//    molecule, then the row molecules' sums into the LDS block.  Deterministic, no
 //    atomics; each K entry is written exactly once, straight into RFP (UPLO='U') or
-//    dense layout.  64-molecule blocks keep the 64-row label sub-tiles ~81% full on
-//    ragged molecules (32-molecule blocks: ~67%).
+//    dense layout.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -33,9 +35,11 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int MB = 64;       // molecules per output block
 constexpr int TS = 64;       // sub-tile edge (packed rows)
-constexpr int KC = 16;       // k chunk staged in LDS
-constexpr int LDA = KC + 1;  // LDS pitch of operand tiles (doubles)
-constexpr int EP = TS + 1;   // LDS pitch of the exp tile
+constexpr int KA = 32;       // row-operand chunk staged in LDS (descriptor columns)
+constexpr int KS = 16;       // column-operand sub-chunk held in registers
+constexpr int LDA = KA + 2;  // LDS pitch of the staged rows (conflict-free 16-byte reads)
+constexpr int PP = TS + 4;   // LDS pitch of the segment sums (<= 64 molecules + 3 window cuts)
+constexpr int AP = MB + 1;   // LDS pitch of the block accumulators
 constexpr int NT = 256;      // threads per workgroup
 
 enum Mode { SYMM_RFP = 0, SYMM_DENSE = 1, RECT = 2 };
@@ -63,123 +67,146 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
 }
 
 
-// One k-chunk (KC) of the wave's Gram fragments: NA 16-row fragments of the row tile
-// times the wave's 16 columns of NTW column tiles, as v_mfma_f64_16x16x4 (A[i][k] from
-// lane i + 16k, B[k][j] from lane j + 16k, D row 4v + lane/16, column lane % 16).
-// The operands of k-step ks+1 are read from LDS while the MFMAs of ks issue.
-template <int NA, int NTW>
-__device__ __forceinline__ void mma_chunk(const double *__restrict__ As,
-                                          const double *__restrict__ B0,
-                                          const double *__restrict__ B1, v4d (&c)[2][4],
-                                          int r16, int bcol, int kq) {
-    if (NTW == 0) return;
-    double a[2][NA], b[2][NTW];
-    auto read = [&](int ks, int s) {
-        const int kk = ks * 4 + kq;
-#pragma unroll
-        for (int x = 0; x < NA; ++x) a[s][x] = As[(x * 16 + r16) * LDA + kk];
-        b[s][0] = B0[bcol * LDA + kk];
-        if (NTW == 2) b[s][NTW - 1] = B1[bcol * LDA + kk];
-    };
-    read(0, 0);
-#pragma unroll
-    for (int ks = 0; ks < KC / 4; ++ks) {
-        if (ks + 1 < KC / 4) read(ks + 1, (ks + 1) & 1);
-#pragma unroll
-        for (int u = 0; u < NTW; ++u)
-#pragma unroll
-            for (int x = 0; x < NA; ++x)
-                c[u][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks & 1][x], b[ks & 1][u],
-                                                               c[u][x], 0, 0, 0);
-    }
-}
-
-// Operand pointers and predicates of one row tile and NTL column tiles.
+// Gram products of one row sub-tile (64 packed rows) with NTL column sub-tiles over the
+// full depth, as v_mfma_f64_16x16x4 (A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, D row 4v + lane/16, column lane % 16).  Wave w owns columns 16w .. 16w+15 of
+// every column tile and multiplies the NA non-empty 16-row fragments with its NTW
+// non-empty column tiles.
+//  * The row tile is shared by the four waves: it is staged global -> registers -> LDS in
+//    chunks of KA columns with two LDS buffers (one barrier per chunk).  The next
+//    chunk is moved in KS-column parts, each stored one sub-chunk after it was loaded.
+//  * A wave's column operands are its own: they go straight from global memory into a
+//    two-slot register ring, one KS sub-chunk ahead.
+//  * Inside a KS sub-chunk, lane group k = lane / 16 reads the 4 consecutive columns
+//    4k .. 4k+3 and k-step ks multiplies column 4k + ks, for both operands (a
+//    permutation of the summation index, so one 32-byte read per operand row).
+// On entry the caller's last barrier has freed both buffers; on exit a barrier has
+// retired every read of them.
 struct GramArgs {
+    // staging row of this thread and column row of this lane, clamped into the tiles:
+    // rows and columns past a tile's end repeat a valid one and their Gram entries are
+    // never read, so every load is unconditional (exact wait counts, no branches)
     const double *pa, *pb0, *pb1;
-    bool va, vb0, vb1;
-    int rep_pad, lofs, r16, bcol, kq;
+    int rep_pad, lofs, r16, kq;
 };
 
-// Gram products of one row sub-tile with NTL column sub-tiles over the full depth.
-// Operands go global -> registers -> LDS with two LDS buffers: while the MFMAs run on
-// chunk k, chunk k+1 is stored into the other buffer and chunk k+2 is loaded into
-// registers, so each chunk costs one barrier.  Every wave stages its share of all NTL
-// tiles but multiplies only its NA x NTW non-empty fragments.  On entry the caller's
-// last barrier has freed both buffers; on exit a barrier has retired every read.
 template <int NTL, int NA, int NTW>
-__device__ void gram_tiles(double *__restrict__ Ops, const GramArgs &g, v4d (&c)[2][4]) {
-    constexpr int LV = KC / 8, BUF = 3 * TS * LDA;
-    double2 ra[LV], rb0[LV], rb1[LV];
-    auto load = [&](int k) {
+__device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c)[2][4]) {
+    constexpr int NSUB = KA / KS;
+    constexpr int NB = NTW > 0 ? NTW : 1;
+    static_assert(KS == 16 && NSUB == 2, "two staged row parts and two ring slots per chunk");
+    double2 ra0, ra1;         // one staged row part: 4 columns of one sub-chunk
+    double2 bq[NSUB][NB][2];  // column ring: sub-chunk gs in slot gs % 2
+#define KF_LOAD_A(k)                                       \
+    do {                                                   \
+        ra0 = *(const double2 *)(g.pa + (k));              \
+        ra1 = *(const double2 *)(g.pa + (k) + 2);          \
+    } while (0)
+#define KF_STORE_A(buf, s)                                 \
+    do {                                                   \
+        *(double2 *)((buf) + g.lofs + (s) * KS) = ra0;     \
+        *(double2 *)((buf) + g.lofs + (s) * KS + 2) = ra1; \
+    } while (0)
+#define KF_LOAD_B(slot, k)                                               \
+    do {                                                                 \
+        bq[slot][0][0] = *(const double2 *)(g.pb0 + (k));                \
+        bq[slot][0][1] = *(const double2 *)(g.pb0 + (k) + 2);            \
+        if (NB == 2) {                                                   \
+            bq[slot][NB - 1][0] = *(const double2 *)(g.pb1 + (k));       \
+            bq[slot][NB - 1][1] = *(const double2 *)(g.pb1 + (k) + 2);   \
+        }                                                                \
+    } while (0)
+    const int nch = g.rep_pad / KA, nsub = nch * NSUB;
 #pragma unroll
-        for (int u = 0; u < LV; ++u) {
-            ra[u] = g.va ? *(const double2 *)(g.pa + k + 2 * u) : double2{0, 0};
-            rb0[u] = g.vb0 ? *(const double2 *)(g.pb0 + k + 2 * u) : double2{0, 0};
-            if (NTL == 2) rb1[u] = g.vb1 ? *(const double2 *)(g.pb1 + k + 2 * u) : double2{0, 0};
-        }
-    };
-    auto store = [&](double *buf) {
-        double *as = buf + g.lofs, *bs0 = buf + TS * LDA + g.lofs;
-        double *bs1 = buf + 2 * TS * LDA + g.lofs;
+    for (int s = 0; s < NSUB; ++s) {
+        KF_LOAD_A(s * KS);
+        KF_STORE_A(Abuf, s);
+    }
+    if (NTW > 0) KF_LOAD_B(0, 0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const double *cur = Abuf + (ch & 1) * (TS * LDA);
+        double *nxt = Abuf + ((ch + 1) & 1) * (TS * LDA);
+        const int cn = min(ch + 1, nch - 1);  // the last chunk re-reads itself (never used)
 #pragma unroll
-        for (int u = 0; u < LV; ++u) {
-            as[2 * u] = ra[u].x;
-            as[2 * u + 1] = ra[u].y;
-            bs0[2 * u] = rb0[u].x;
-            bs0[2 * u + 1] = rb0[u].y;
-            if (NTL == 2) {
-                bs1[2 * u] = rb1[u].x;
-                bs1[2 * u + 1] = rb1[u].y;
+        for (int s = 0; s < NSUB; ++s) {
+            const int gs = ch * NSUB + s;
+            // Issue order is pinned (sched_barrier) and every load and store is
+            // unconditional, so the wait counts are exact: each load is consumed one
+            // sub-chunk after it is issued and its latency hides behind these MFMAs.
+            // In the last chunk the parts land in the idle buffer and are never read.
+            __builtin_amdgcn_sched_barrier(0);
+            if (s > 0) KF_STORE_A(nxt, s - 1);
+            KF_LOAD_A(cn * KA + s * KS);
+            if (NTW > 0) KF_LOAD_B((s + 1) % NSUB, min(gs + 1, nsub - 1) * KS);
+            __builtin_amdgcn_sched_barrier(0);
+            if (NTW == 0) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {  // k-steps 2h, 2h+1
+                double2 a[NA];
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+                    a[x] = *(const double2 *)(cur + (x * 16 + g.r16) * LDA + s * KS + 4 * g.kq + 2 * h);
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const double b = kk ? bq[s][u][h].y : bq[s][u][h].x;
+#pragma unroll
+                        for (int x = 0; x < NA; ++x)
+                            c[u][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(kk ? a[x].y : a[x].x, b,
+                                                                           c[u][x], 0, 0, 0);
+                    }
+                }
             }
         }
-    };
-    const int nch = g.rep_pad / KC;
-    load(0);
-    store(Ops);
-    if (nch > 1) load(KC);
-    __syncthreads();
-    for (int kc = 0; kc < nch; ++kc) {
-        double *cur = Ops + (kc & 1) * BUF;
-        if (kc + 1 < nch) {
-            store(Ops + ((kc + 1) & 1) * BUF);
-            if (kc + 2 < nch) load((kc + 2) * KC);
-        }
-        mma_chunk<NA, NTW>(cur, cur + TS * LDA, cur + 2 * TS * LDA, c, g.r16, g.bcol, g.kq);
+        __builtin_amdgcn_sched_barrier(0);
+        KF_STORE_A(nxt, NSUB - 1);
         __syncthreads();
     }
+#undef KF_LOAD_A
+#undef KF_STORE_A
+#undef KF_LOAD_B
 }
 
 template <int NTL, int NA>
-__device__ __forceinline__ void gram_ntw(double *Ops, const GramArgs &g, int ntw,
+__device__ __forceinline__ void gram_ntw(double *Abuf, const GramArgs &g, int ntw,
                                          v4d (&c)[2][4]) {
-    if (ntw == 0) gram_tiles<NTL, NA, 0>(Ops, g, c);
-    else if (NTL == 1 || ntw == 1) gram_tiles<NTL, NA, 1>(Ops, g, c);
-    else gram_tiles<NTL, NA, NTL>(Ops, g, c);
+    if (ntw == 0) gram_tiles<NTL, NA, 0>(Abuf, g, c);
+    else if (NTL == 1 || ntw == 1) gram_tiles<NTL, NA, 1>(Abuf, g, c);
+    else gram_tiles<NTL, NA, NTL>(Abuf, g, c);
 }
 
 template <int NTL>
-__device__ __forceinline__ void gram_dispatch(double *Ops, const GramArgs &g, int na16, int ntw,
+__device__ __forceinline__ void gram_dispatch(double *Abuf, const GramArgs &g, int na16, int ntw,
                                               v4d (&c)[2][4]) {
     switch (na16) {
-        case 1: gram_ntw<NTL, 1>(Ops, g, ntw, c); break;
-        case 2: gram_ntw<NTL, 2>(Ops, g, ntw, c); break;
-        case 3: gram_ntw<NTL, 3>(Ops, g, ntw, c); break;
-        default: gram_ntw<NTL, 4>(Ops, g, ntw, c); break;
+        case 1: gram_ntw<NTL, 1>(Abuf, g, ntw, c); break;
+        case 2: gram_ntw<NTL, 2>(Abuf, g, ntw, c); break;
+        case 3: gram_ntw<NTL, 3>(Abuf, g, ntw, c); break;
+        default: gram_ntw<NTL, 4>(Abuf, g, ntw, c); break;
     }
 }
 
-// Output block: 64 x 64 molecule pairs per work-group.  The block's K entries live in
-// registers: thread t owns column molecule b = t % 64 and row molecules
-// 16 (t / 64) .. +15.  Per label, the row sub-tile sr is multiplied with two column
-// sub-tiles at once (the row operand is staged once for both, 3 operand panels per 2
-// output tiles instead of 4); wave w computes columns 16w .. 16w+15 of both tiles for
-// every non-empty 16-row fragment.  Each tile's exp tile is then reduced to row-wise run
-// sums over column molecules (a run is the columns of one molecule; four threads per
-// row, the partial sums of a run that crosses a 16-column quarter carried across the
-// four lanes), and each thread adds the rows of its 16 row molecules for its column
-// molecule's run.  The molecule of a packed row is src[row] / max_atoms
-// (src = a * max_atoms + i, see local_plan.hpp).
+// v + (value of the lane D places lower in the same 16-lane row), when `take`: one step
+// of a segmented inclusive scan along the 16 columns a row of MFMA output spans.
+template <int D>
+__device__ __forceinline__ double scan_step(double v, bool take) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int slo = __builtin_amdgcn_update_dpp(0, lo, 0x110 + D, 0xF, 0xF, true);
+    const int shi = __builtin_amdgcn_update_dpp(0, hi, 0x110 + D, 0xF, 0xF, true);
+    return take ? v + __hiloint2double(shi, slo) : v;
+}
+
+// Output block: 64 x 64 molecule pairs per work-group, accumulated in LDS (each entry
+// owned by one thread: column molecule t % 64, row molecules 16 (t / 64) .. +15).  Per
+// label, the row sub-tile sr is multiplied with two column sub-tiles at once (the row
+// operand is staged once for both).  Epilogue per tile, in registers: exp of every
+// Gram entry, then a segmented scan along each wave's 16 columns sums the columns of
+// one molecule (a window segment: a molecule's columns inside one 16-column window);
+// the segment tails go to LDS as partial sums P[row][segment], and each thread adds,
+// for its column molecule's segments, the rows of its row molecules.  The molecule of a
+// packed row is src[row] / max_atoms (src = a * max_atoms + i, see local_plan.hpp).
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
@@ -187,13 +214,15 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
     int ma2, int L, int nm1, int nm2, int rep_pad, double cexp, const int2 *__restrict__ order,
     double *__restrict__ out) {
-    // two operand buffers (row tile + two column tiles each); the exp tile and the run
-    // sums of the epilogue reuse the same memory once the Gram products are done
-    constexpr int OPS = 2 * 3 * TS * LDA, EPI = 2 * TS * EP;
+    // two row-operand buffers; the segment partial sums of the epilogue reuse the same
+    // memory once the Gram products are done
+    constexpr int OPS = 2 * TS * LDA, EPI = TS * PP;
     __shared__ double Sh[OPS > EPI ? OPS : EPI];
-    double *const Ops = Sh, *const Es = Sh, *const Rs = Sh + TS * EP;
+    __shared__ double accL[MB * AP];
+    double *const Abuf = Sh, *const P = Sh;
     __shared__ double n2r[TS], n2c[2][TS];
-    __shared__ int rowmol[TS], colmol[2][TS], colrun[2][TS + 1], run_of_mol[2][MB];
+    __shared__ int rowmol[TS], colmol[2][TS], colseg[2][TS], colinfo[2][TS];
+    __shared__ int segf[2][MB], segl[2][MB], rstart[TS + 1], nrr_s;
     __shared__ int mrA[MB + 1], mrB[MB + 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -207,16 +236,13 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
     const bool diag = (MODE != RECT) && (A == B);
     const int ob = tid & (MB - 1), oa0 = (tid / MB) * (MB / 4);
-    const int lrow = tid >> 2;
+    const int lrow = tid >> 2, bcol = wave * 16 + (lane & 15);
     GramArgs ga;
     ga.rep_pad = rep_pad;
-    ga.lofs = lrow * LDA + (tid & 3) * (KC / 4);
+    ga.lofs = lrow * LDA + (tid & 3) * 4;
     ga.r16 = lane & 15;
-    ga.bcol = wave * 16 + (lane & 15);
     ga.kq = lane >> 4;
-    double acc[MB / 4];
-#pragma unroll
-    for (int k = 0; k < MB / 4; ++k) acc[k] = 0.0;
+    for (int k = tid; k < MB * AP; k += NT) accL[k] = 0.0;
 
     for (int l = 0; l < L; ++l) {
         __syncthreads();
@@ -236,23 +262,41 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                 c0[1] = c0[0] + TS;
                 cn[1] = ntile == 2 ? min(TS, rb1 - c0[1]) : 0;
                 __syncthreads();  // the previous tiles' epilogue is done with Sh and the maps
-                if (wave == 0) {  // molecule and |x|^2 of every row
+                if (wave == 0) {  // molecule and |x|^2 of every row; the row molecules' ranges
                     const bool v = lane < rn;
-                    rowmol[lane] = v ? src1[r0 + lane] / ma1 - a0 : -1;
+                    const int m = v ? src1[r0 + lane] / ma1 - a0 : -1;
+                    rowmol[lane] = m;
                     n2r[lane] = v ? n2a[r0 + lane] : 0.0;
-                } else if (wave <= 2) {  // molecule, run and |x|^2 of every column of tile wave-1
+                    const int prev = __shfl_up(m, 1, 64);
+                    const bool rh = v && (lane == 0 || prev != m);
+                    const unsigned long long rheads = __ballot(rh);
+                    if (rh) rstart[__popcll(rheads & ((2ull << lane) - 1)) - 1] = lane;
+                    if (lane == 0) {
+                        nrr_s = __popcll(rheads);
+                        rstart[__popcll(rheads)] = rn;
+                    }
+                } else if (wave <= 2) {  // column maps of tile wave-1
                     const int u = wave - 1;
                     const bool v = lane < cn[u];
-                    run_of_mol[u][lane] = -1;  // same wave: ordered before the writes below
+                    segf[u][lane] = -1;  // same wave: ordered before the writes below
                     const int m = v ? src2[c0[u] + lane] / ma2 - b0 : -1;
                     n2c[u][lane] = v ? n2b[c0[u] + lane] : 0.0;
-                    const int prev = __shfl_up(m, 1, 64);
-                    const bool head = v && (lane == 0 || prev != m);
-                    const unsigned long long heads = __ballot(head);
-                    const int run = __popcll(heads & ((2ull << lane) - 1)) - 1;
+                    const int prev = __shfl_up(m, 1, 64), next = __shfl_down(m, 1, 64);
+                    const bool mhead = v && (lane == 0 || prev != m);
+                    const bool mtail = v && (lane == 63 || next != m);
+                    // window segments: a molecule's columns inside one 16-column window
+                    const bool shead = v && (mhead || (lane & 15) == 0);
+                    const unsigned long long heads = __ballot(shead);
+                    const unsigned long long upto = heads & ((2ull << lane) - 1);
+                    const int seg = __popcll(upto) - 1;
+                    const int dist = v ? lane - (63 - __clzll(upto)) : 0;
+                    const bool stail = v && ((lane & 15) == 15 || lane + 1 >= cn[u] ||
+                                             ((heads >> (lane + 1)) & 1ull));
                     colmol[u][lane] = m;
-                    colrun[u][lane] = run;
-                    if (head) run_of_mol[u][m] = run;
+                    colseg[u][lane] = seg;
+                    colinfo[u][lane] = dist | (stail ? 256 : 0);
+                    if (mhead) segf[u][m] = seg;
+                    if (mtail) segl[u][m] = seg;
                 }
 
                 // ---- Gram sub-tiles on MFMA, the row operand staged once ----
@@ -261,116 +305,84 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
                     for (int x = 0; x < 4; ++x) c[u][x] = (v4d){0, 0, 0, 0};
-                ga.pa = Xp1 + (size_t)(r0 + lrow) * rep_pad + (tid & 3) * (KC / 4);
-                ga.pb0 = Xp2 + (size_t)(c0[0] + lrow) * rep_pad + (tid & 3) * (KC / 4);
-                ga.pb1 = ga.pb0 + (size_t)TS * rep_pad;
-                ga.va = lrow < rn;
-                ga.vb0 = lrow < cn[0];
-                ga.vb1 = lrow < cn[1];
+                ga.pa = Xp1 + (size_t)(r0 + min(lrow, rn - 1)) * rep_pad + (tid & 3) * 4;
+                ga.pb0 = Xp2 + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad + 4 * ga.kq;
+                ga.pb1 = ntile == 2 ? Xp2 + (size_t)(c0[1] + min(bcol, cn[1] - 1)) * rep_pad + 4 * ga.kq
+                                    : ga.pb0;
                 const int na16 = (rn + 15) >> 4;
                 const int ntw = (wave * 16 < cn[1]) ? 2 : (wave * 16 < cn[0] ? 1 : 0);
                 if (ntile == 2)
-                    gram_dispatch<2>(Ops, ga, na16, ntw, c);
+                    gram_dispatch<2>(Abuf, ga, na16, ntw, c);
                 else
-                    gram_dispatch<1>(Ops, ga, na16, ntw, c);
+                    gram_dispatch<1>(Abuf, ga, na16, ntw, c);
 
-                // ---- per tile: exp into LDS, row runs, register accumulation ----
+                // ---- per tile: exp, segment sums in registers, row molecules ----
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     if (u == 1 && ntile == 1) break;
-                    const int cnu = cn[u];
                     const bool dbl_self = diag && sc0 + u > sr;  // mirror sub-tile not visited
-                    if (u == 1) __syncthreads();  // tile 0's stage 2 is done with Es and Rs
+                    if (u == 1) __syncthreads();  // tile 0's row sums are done with P
                     {
-                        const int col = wave * 16 + (lane & 15);
+                        const int col = bcol;
+                        const bool colok = col < cn[u];
                         const double nc = n2c[u][col];
-                        const int cm = colmol[u][col];
+                        const int cm = colmol[u][col], seg = colseg[u][col];
+                        const int info = colinfo[u][col], dist = info & 255;
+                        const bool stail = (info >> 8) & 1;
 #pragma unroll
                         for (int x = 0; x < 4; ++x) {
                             const v4d cv = c[u][x];
 #pragma unroll
                             for (int v = 0; v < 4; ++v) {
                                 const int row = x * 16 + mfma16_row_f64(lane, v);
+                                const bool ok = colok && row < rn;
                                 double e = 0.0;
-                                if (row < rn && col < cnu) {
+                                if (ok) {
                                     e = exp((n2r[row] + nc - 2.0 * cv[v]) * cexp);
                                     if (dbl_self && rowmol[row] == cm) e *= 2.0;
                                 }
-                                Es[row * EP + col] = e;
+                                e = scan_step<1>(e, dist >= 1);
+                                e = scan_step<2>(e, dist >= 2);
+                                e = scan_step<4>(e, dist >= 4);
+                                e = scan_step<8>(e, dist >= 8);
+                                if (ok && stail) P[row * PP + seg] = e;
                             }
                         }
                     }
                     __syncthreads();
-                    // stage 1: run sums of row i over column molecules, four threads per
-                    // row (16 columns each); a run's partial sum is carried to the next
-                    // quarter while the run continues there
-                    {
-                        const int i = tid >> 2, q = tid & 3;
-                        const int *cr = colrun[u];
-                        const double *er = Es + i * EP;
-                        double s = 0.0, first = 0.0;
-                        int rfirst = -1;
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) {
-                            const int col = q * 16 + j;
-                            if (col < cnu) {
-                                s += er[col];
-                                const int rj = cr[col];
-                                if (col == cnu - 1 || cr[col + 1] != rj) {  // run ends here
-                                    if (rfirst < 0) {
-                                        first = s;
-                                        rfirst = rj;
-                                    } else if (i < rn) {
-                                        Rs[i * EP + rj] = s;
-                                    }
-                                    s = 0.0;
-                                }
-                            }
-                        }
-                        // carry: the open run at the end of quarter q-1 (plus its own
-                        // carry-in when it spans that whole quarter) continues here
-                        double cin = 0.0, cout = s;
-#pragma unroll
-                        for (int step = 1; step < 4; ++step) {
-                            const double prev = __shfl_up(cout, 1, 64);
-                            if (q == step) {
-                                cin = prev;
-                                cout = rfirst < 0 ? cin + s : s;
-                            }
-                        }
-                        if (rfirst >= 0 && i < rn) Rs[i * EP + rfirst] = first + cin;
-                    }
-                    __syncthreads();
-                    // stage 2: each thread adds the rows of its row molecules for its column
-                    const int run = ob < nb ? run_of_mol[u][ob] : -1;
-                    if (run >= 0) {
-#pragma unroll
-                        for (int k = 0; k < MB / 4; ++k) {
-                            const int a = oa0 + k;
-                            if (a < na) {
-                                const int rs = max(mrA[a], r0) - r0, re = min(mrA[a + 1], r0 + rn) - r0;
-                                double sum = 0.0;
-                                for (int i = rs; i < re; ++i) sum += Rs[i * EP + run];
-                                acc[k] += sum;
-                            }
+                    // thread (wave w, lane b) adds, for column molecule b, every 4th row
+                    // molecule of the tile (from w) over the rows of that molecule and the
+                    // window segments of b; within a tile each (row, column) molecule pair
+                    // has one owner
+                    const int s0 = ob < nb ? segf[u][ob] : -1;
+                    if (s0 >= 0) {
+                        const int s1 = segl[u][ob], nrr = nrr_s;
+                        for (int j = wave; j < nrr; j += 4) {
+                            const int rs = rstart[j], re = rstart[j + 1];
+                            double sum = 0.0;
+                            for (int i = rs; i < re; ++i)
+                                for (int sg = s0; sg <= s1; ++sg) sum += P[i * PP + sg];
+                            accL[rowmol[rs] * AP + ob] += sum;
                         }
                     }
                 }
             }
         }
     }
+    __syncthreads();
     // ---- write the block ----
 #pragma unroll
     for (int k = 0; k < MB / 4; ++k) {
         const int a = oa0 + k, b = ob;
         if (a >= na || b >= nb) continue;
+        const double val = accL[a * AP + b];
         if (MODE == SYMM_RFP) {
-            if (!diag || a <= b) out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = acc[k];
+            if (!diag || a <= b) out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = val;
         } else if (MODE == RECT) {
-            out[(size_t)(a0 + a) * nm2 + b0 + b] = acc[k];
+            out[(size_t)(a0 + a) * nm2 + b0 + b] = val;
         } else if (!diag || a <= b) {
-            out[(size_t)(a0 + a) * nm1 + b0 + b] = acc[k];
-            out[(size_t)(b0 + b) * nm1 + a0 + a] = acc[k];
+            out[(size_t)(a0 + a) * nm1 + b0 + b] = val;
+            out[(size_t)(b0 + b) * nm1 + a0 + a] = val;
         }
     }
 }
@@ -385,7 +397,7 @@ struct DevicePacked {
 int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, int ws_plan,
                     hipStream_t stream, DevicePacked &d) {
     d.rows = p.rows;
-    d.rep_pad = (rep + KC - 1) / KC * KC;
+    d.rep_pad = (rep + KA - 1) / KA * KA;
     const size_t mr_n = p.mol_row.size(), src_n = p.src.size();
     char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * (mr_n + src_n), stream));
     const size_t pack_bytes = sizeof(double) * ((size_t)p.rows * d.rep_pad + p.rows + 64);
