@@ -350,6 +350,189 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
             }
 }
 
+// Trailing update of everything right of an outer panel on 128 x 128 super-tiles:
+//   A_IK -= sum_{J in [Jlo, Jhi)} A_IJ A_KJ^T   for Kbeg <= K <= I < nblk.
+// Blocks are paired inside each RFP region (pairs never straddle n/2; a region's odd
+// last block pairs with nothing), and a work-group owns (I-pair x K-pair): wave
+// (wi, wj) computes block (I_wi, K_wj) as 4 x 4 MFMA fragments, so each staged panel
+// row block feeds two waves (half the panel traffic of one-block tiles).  Panels
+// stream through LDS in 16-column chunks with two buffers (one barrier per chunk).
+// The region of each pair fixes its load pattern at compile time (coalesced along
+// the contiguous axis); every global load is unconditional (rows clamped into the
+// block, columns past a panel's width read a valid column and are zeroed), so the
+// wait counts are exact.  In the column-major region the waves compute C^T so that
+// consecutive lanes touch contiguous rows.
+constexpr int SK = 16;       // chunk depth
+constexpr int SP = SK + 1;   // LDS pitch
+
+struct PairList {  // pairs of blocks from Kbeg: lower region first, then upper
+    int kbeg, npl, u0, np;
+    __device__ int first(int p) const { return p < npl ? kbeg + 2 * p : u0 + 2 * (p - npl); }
+};
+inline PairList make_pairs(const Part &P, int Kbeg) {
+    PairList L;
+    L.kbeg = Kbeg;
+    L.npl = Kbeg < P.nlo ? (P.nlo - Kbeg + 1) / 2 : 0;
+    L.u0 = Kbeg > P.nlo ? Kbeg : P.nlo;
+    L.np = L.npl + (P.nblk - L.u0 + 1) / 2;
+    return L;
+}
+
+template <typename T, bool CMI, bool CMK>
+__device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int Jlo, int Jhi,
+                                           const int (&bI)[2], const int (&bK)[2], T *Lb) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int I = bI[wi], K = bK[wj];
+    const bool active = I >= 0 && K >= 0 && I >= K;
+    const int rni = active ? P.width(I) : 0, rnk = active ? P.width(K) : 0;
+    const bool diagt = I == K;
+    constexpr bool trans = CMI;  // C rows are I rows: column-major region -> compute C^T
+    const size_t s = (P.n & 1) ? P.n : P.n + 1;
+    typedef typename Vec4<T>::type V4;
+    V4 acc[4][4];
+    {
+        const Tile tc = tile_at(P, P.start(active ? I : 0), P.start(active ? K : 0));
+        const T *cb = a + tc.base;
+        const int trs = (int)tc.rs, tcs = (int)tc.cs;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = i * 16 + mfma16_row<T>(lane, v), y = j * 16 + (lane & 15);
+                    const int row = trans ? y : x, col = trans ? x : y;
+                    T c = T(0);
+                    if (row < rni && col < rnk && (!diagt || row >= col)) c = cb[row * trs + col * tcs];
+                    acc[i][j][v] = c;
+                }
+    }
+    // staging: slot b = 0,1 (I blocks), 2,3 (K blocks), 4 elements per thread each.
+    // Column-major region: element (r, c) at k + 1 + r0 + r + c * s; row-major region:
+    // (r0 - k + r) * s + c (c a global column).  Thread -> (r, c): column-major
+    // r = tid % 64, c = tid / 64 + 4e; row-major r = tid / 16 + 16e, c = tid % 16.
+    const T *src[4];
+    int rmax[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int blk = b < 2 ? bI[b] : bK[b - 2];
+        const int bb = blk >= 0 ? blk : (b < 2 ? bI[0] : bK[0]);
+        const bool cm = b < 2 ? CMI : CMK;
+        const int r0 = P.start(bb);
+        rmax[b] = P.width(bb) - 1;
+        src[b] = cm ? a + (size_t)(P.k + 1 + r0) : a + (size_t)(r0 - P.k) * s;
+    }
+    const int rc = tid & 63, cc_c = tid >> 6, rr_r = tid >> 4, cc_r = tid & 15;
+    T st[4][4];
+#define KF_SYRK_LOAD(J, kc)                                                                       \
+    do {                                                                                          \
+        const int c0_ = P.start(J), wl_ = P.width(J) - 1;                                         \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                           \
+            const bool cm_ = b < 2 ? CMI : CMK;                                                   \
+            _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                       \
+                const int r_ = cm_ ? rc : rr_r + 16 * e;                                          \
+                const int c_ = (kc) * SK + (cm_ ? cc_c + 4 * e : cc_r);                           \
+                const int rcl = min(r_, rmax[b]), ccl = c0_ + min(c_, wl_);                       \
+                const T v_ = cm_ ? src[b][rcl + (size_t)ccl * s] : src[b][(size_t)rcl * s + ccl]; \
+                st[b][e] = c_ <= wl_ ? v_ : T(0);                                                 \
+            }                                                                                     \
+        }                                                                                         \
+    } while (0)
+#define KF_SYRK_STORE(buf)                                                                        \
+    do {                                                                                          \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                           \
+            const bool cm_ = b < 2 ? CMI : CMK;                                                   \
+            _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                       \
+                const int r_ = cm_ ? rc : rr_r + 16 * e;                                          \
+                const int c_ = cm_ ? cc_c + 4 * e : cc_r;                                         \
+                (buf)[(b * NB + r_) * SP + c_] = st[b][e];                                        \
+            }                                                                                     \
+        }                                                                                         \
+    } while (0)
+    const int nch = (Jhi - Jlo) * (NB / SK);
+    KF_SYRK_LOAD(Jlo, 0);
+    KF_SYRK_STORE(Lb);
+    if (nch > 1) KF_SYRK_LOAD(Jlo, 1);
+    __syncthreads();
+    // rows of the computed tile (X) and its columns (Y) inside a buffer
+    const int xo = trans ? (2 + wj) * NB : wi * NB, yo = trans ? wi * NB : (2 + wj) * NB;
+    for (int ch = 0; ch < nch; ++ch) {
+        const T *cur = Lb + (ch & 1) * (4 * NB * SP);
+        if (ch + 1 < nch) {
+            KF_SYRK_STORE(Lb + ((ch + 1) & 1) * (4 * NB * SP));
+            const int c2 = min(ch + 2, nch - 1);
+            KF_SYRK_LOAD(Jlo + c2 / (NB / SK), c2 % (NB / SK));
+        }
+        if (active) {
+#pragma unroll
+            for (int k0 = 0; k0 < SK; k0 += 4) {
+                const int kk = k0 + (lane >> 4);
+                T av[4], bv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) av[i] = cur[(xo + i * 16 + (lane & 15)) * SP + kk];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bv[j] = cur[(yo + j * 16 + (lane & 15)) * SP + kk];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(-av[i], bv[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+#undef KF_SYRK_LOAD
+#undef KF_SYRK_STORE
+    if (!active) return;
+    {
+        // recompute the addresses here rather than keep 64 of them live through the loop
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const Tile tc = tile_at(P, P.start(I), P.start(K));
+        T *cb = a + tc.base;
+        const int trs = (int)tc.rs, tcs = (int)tc.cs;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = i * 16 + mfma16_row<T>(ln, v), y = j * 16 + (ln & 15);
+                    const int row = trans ? y : x, col = trans ? x : y;
+                    if (row < rni && col < rnk && (!diagt || row >= col)) cb[row * trs + col * tcs] = acc[i][j][v];
+                }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void syrk2_range(T *a, Part P, PairList L, int Jlo, int Jhi,
+                                                      const int *info) {
+    __shared__ T Lb[2 * 4 * NB * SP];  // per buffer: two I blocks then two K blocks
+    if (*info) return;
+    // decode t -> (q, SI): pair column q has np - q super-tiles SI = q..np-1
+    const long t = blockIdx.x;
+    const long r = L.np;
+    long q = (long)((2.0 * r + 1.0 - sqrt((2.0 * r + 1.0) * (2.0 * r + 1.0) - 8.0 * (double)t)) * 0.5);
+    if (q < 0) q = 0;
+    while (q > 0 && q * r - q * (q - 1) / 2 > t) --q;
+    while ((q + 1) * r - (q + 1) * q / 2 <= t) ++q;
+    const int SI = (int)q + (int)(t - (q * r - q * (q - 1) / 2));
+    if (q >= r || SI >= r) return;
+    int bI[2], bK[2];
+    bI[0] = L.first(SI);
+    bI[1] = (bI[0] + 1 < P.nblk && bI[0] + 1 != P.nlo) ? bI[0] + 1 : -1;  // pairs stay in a region
+    bK[0] = L.first((int)q);
+    bK[1] = (bK[0] + 1 < P.nblk && bK[0] + 1 != P.nlo) ? bK[0] + 1 : -1;
+    const bool cmI = bI[0] < P.nlo, cmK = bK[0] < P.nlo;
+    if (cmI) {
+        if (cmK) syrk2_body<T, true, true>(a, P, Jlo, Jhi, bI, bK, Lb);
+        else syrk2_body<T, true, false>(a, P, Jlo, Jhi, bI, bK, Lb);
+    } else {
+        if (cmK) syrk2_body<T, false, true>(a, P, Jlo, Jhi, bI, bK, Lb);
+        else syrk2_body<T, false, false>(a, P, Jlo, Jhi, bI, bK, Lb);
+    }
+}
+
 inline long range_tiles(const Part &P, int Kbeg, int Kend) {
     long n = 0;
     for (int K = Kbeg; K < Kend; ++K) n += P.nblk - K;
@@ -508,9 +691,10 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
             }
         }
         if (J1 < P.nblk) {  // everything right of the outer panel, 256-deep
-            const long nt = range_tiles(P, J1, P.nblk);
-            hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, J0,
-                               J1, J1, P.nblk, info);
+            const PairList L = make_pairs(P, J1);
+            const long nt = (long)L.np * (L.np + 1) / 2;
+            hipLaunchKernelGGL(syrk2_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, L,
+                               J0, J1, info);
         }
     }
     KF_HIP_TRY(hipGetLastError());

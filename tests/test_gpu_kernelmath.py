@@ -40,7 +40,7 @@ def test_golden(km):
     assert rel_err(a, g["alpha"]) < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 128, 129, 200, 511, 1001])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 128, 129, 200, 511, 1001, 1500, 2311])
 def test_cho_solve_rfp_and_factor(km, n):
     K = spd(n, n)
     y = np.random.default_rng(1).standard_normal(n)
@@ -102,10 +102,10 @@ def test_kernel_layout_is_full_to_rfp_lower(km):
     np.testing.assert_array_equal(km.full_to_rfp(K, uplo="L"), to_rfp(K))
 
 
-def test_fp32_potrf_potrs_c_abi():
+@pytest.mark.parametrize("n", [300, 1300])
+def test_fp32_potrf_potrs_c_abi(n):
     from kernelforge_amd import capi
 
-    n = 300
     K = spd(n, 4, shift=2.0)
     arf = to_rfp(K)
     y = np.random.default_rng(2).standard_normal(n)
