@@ -60,6 +60,12 @@ enum WsSlot {
 };
 void *workspace(int slot, size_t bytes, hipStream_t stream);
 
+// Per-device auxiliary stream and events (created once, kept for the process) for calls
+// that fork work onto a second stream and join back through events.
+constexpr int kAuxEvents = 4;
+hipStream_t aux_stream();
+hipEvent_t aux_event(int i);
+
 // Pinned host staging for small metadata round trips.
 void *host_staging(int slot, size_t bytes);
 

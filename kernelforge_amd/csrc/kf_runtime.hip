@@ -57,6 +57,36 @@ void *workspace(int slot, size_t bytes, hipStream_t stream) {
     return p;
 }
 
+// One extra stream and a few events per device, created on first use and kept for the
+// process: a call that overlaps two dependency chains (the Cholesky look-ahead) forks
+// onto the aux stream and joins back through events.  Default priority, non-blocking
+// (ordering with the caller's stream is only through the events).
+hipStream_t aux_stream() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    static hipStream_t streams[kMaxDevices] = {};
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        streams[dev] = nullptr;
+    }
+    return streams[dev];
+}
+
+hipEvent_t aux_event(int i) {
+    int dev = 0;
+    if (i < 0 || i >= kAuxEvents) return nullptr;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    static hipEvent_t events[kMaxDevices][kAuxEvents] = {};
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipEvent_t &e = events[dev][i];
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+    }
+    return e;
+}
+
 void *host_staging(int slot, size_t bytes) {
     if (bytes == 0) bytes = 64;
     std::lock_guard<std::mutex> lock(g_ws_mutex);

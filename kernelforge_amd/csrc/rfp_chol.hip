@@ -77,15 +77,21 @@ __device__ inline Tile tile_at(const Part &P, int r0, int c0) {
     return t;
 }
 
-// Load a (rn x cn) tile into LDS dst[r*TP + c]; coalesced along the contiguous axis.
-template <typename T, bool LOWER_ONLY>
+// Load a (rn x cn) tile into LDS dst[r*TP + c] with NT threads; coalesced along the
+// contiguous axis.  The trip count is a compile-time constant and the loop is
+// unrolled so that the loads are in flight together (a one-at-a-time loop pays the
+// memory latency NB*NB/NT times).
+template <typename T, bool LOWER_ONLY, int NT = 256>
 __device__ inline void load_tile(const T *__restrict__ a, const Tile &t, int rn, int cn, T *dst) {
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+#pragma unroll 16
+    for (int e = 0; e < NB * NB / NT; ++e) {
+        const int idx = threadIdx.x + NT * e;
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        T v = T(0);
-        if (r < rn && c < cn && (!LOWER_ONLY || r >= c)) v = a[t.base + r * t.rs + c * t.cs];
-        dst[r * TP + c] = v;
+        // unconditional load of a clamped (valid) element, then select: a load under a
+        // branch is waited for before the next one is issued
+        const T v = a[t.base + min(r, rn - 1) * t.rs + min(c, cn - 1) * t.cs];
+        dst[r * TP + c] = (r < rn && c < cn && (!LOWER_ONLY || r >= c)) ? v : T(0);
     }
 }
 template <typename T, bool LOWER_ONLY>
@@ -101,42 +107,6 @@ template <typename T>
 __global__ void add_diag(T *a, int n, double l2) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) a[rfp_lower((size_t)n, j, j)] += T(l2);
-}
-
-// In-place Cholesky of the lower triangle of a w x w LDS tile L (pitch TP), 256
-// threads on a 16 x 16 grid, one barrier per column.  Phase j reads the unscaled
-// column j and the lower trailing matrix and writes the trailing update
-// L[i][c] -= L[i][j] L[c][j] / L[j][j] (c > j), the scaled column into the unused upper
-// triangle (L[j][i]) and the pivot root into dg[j]; nothing read in a phase is written
-// in it by another thread.  The lower triangle is restored from the upper at the end.
-// Returns 0, or the 1-based column whose pivot was not positive (uniform over threads).
-template <typename T>
-__device__ inline int chol_tile(T *L, T *dg, int w) {
-    const int ti = threadIdx.x >> 4, tc = threadIdx.x & 15;
-    for (int j = 0; j < w; ++j) {
-        const T piv = L[j * TP + j];
-        if (!(piv > T(0))) return j + 1;  // also catches NaN
-        const T d = sqrt(piv), rd = T(1) / d, rp = T(1) / piv;
-        for (int i = j + ti; i < w; i += 16) {
-            const T lij = L[i * TP + j];
-            for (int c = j + tc; c <= i; c += 16) {
-                if (c > j)
-                    L[i * TP + c] -= lij * L[c * TP + j] * rp;
-                else if (i > j)
-                    L[j * TP + i] = lij * rd;
-                else
-                    dg[j] = d;
-            }
-        }
-        __syncthreads();
-    }
-    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
-        const int i = idx / w, c = idx - i * w;
-        if (i > c) L[i * TP + c] = L[c * TP + i];
-        else if (i == c) L[i * TP + i] = dg[i];
-    }
-    __syncthreads();
-    return 0;
 }
 
 // Li = L^-1 for a lower-triangular w x w LDS tile (pitch TP), right-looking
@@ -181,28 +151,77 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
     __syncthreads();
 }
 
-// POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB].
+// POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB], one wave.
+// Lane i keeps row i of the tile in registers.  Cholesky, right-looking: per column j
+// the pivot comes from lane j (readlane), every lane scales its L[i][j] and publishes
+// it in LDS, and updates its row with the column read back (broadcast reads; one
+// wave, so LDS order needs no barrier).  Inverse: lane c keeps column c of L^-1 and
+// runs forward substitution over the rows of L (broadcast reads).  A partial tile
+// (w < 64) is padded with the identity, which keeps both factors block diagonal.
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
-    __shared__ T L[NB * TP];
-    __shared__ T Li[NB * TP];
-    __shared__ T dg[NB];
+__device__ __forceinline__ T lane_value(T v, int l);
+template <>
+__device__ __forceinline__ double lane_value<double>(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+template <>
+__device__ __forceinline__ float lane_value<float>(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
+    __shared__ T S[NB * TP];
+    __shared__ T col[NB];
     if (*info) return;
+    const int i = threadIdx.x;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
-    load_tile<T, true>(a, t, w, w, L);
+    load_tile<T, true, 64>(a, t, w, w, S);
     __syncthreads();
-    const int fail = chol_tile<T>(L, dg, w);
+    if (i >= w) S[i * TP + i] = T(1);
+    __syncthreads();
+    T r[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) r[c] = c <= i ? S[i * TP + c] : T(0);
+    int fail = 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const T piv = lane_value<T>(r[j], j);
+        if (!(piv > T(0))) {  // uniform: also catches NaN
+            fail = j + 1;
+            break;
+        }
+        const T d = sqrt(piv), rd = T(1) / d;
+        r[j] = i == j ? d : r[j] * rd;
+        col[i] = r[j];  // read back below by the same wave: LDS keeps its order
+#pragma unroll
+        for (int k = j + 1; k < NB; ++k) r[k] -= r[j] * col[k];
+    }
     if (fail) {
-        if (threadIdx.x == 0) atomicCAS(info, 0, s0 + fail);
+        if (i == 0 && fail <= w) atomicCAS(info, 0, s0 + fail);
         return;
     }
-    tri_inverse<T, T>(L, Li, dg, w);
-    store_tile<T, true>(a, t, w, w, L);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-        const int r = idx / NB, c = idx % NB;
-        inv[idx] = (r < w && c < w) ? Li[r * TP + c] : T(0);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) S[i * TP + c] = c <= i ? r[c] : T(0);
+    __syncthreads();
+    store_tile<T, true>(a, t, w, w, S);
+    // column c = i of L^-1: x[q] = (delta_qc - sum_{k<q} L[q][k] x[k]) / L[q][q]
+    T x[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        T s0a = q == i ? T(1) : T(0), s1a = T(0);
+#pragma unroll
+        for (int k = 0; k < q; k += 2) {
+            s0a -= S[q * TP + k] * x[k];
+            if (k + 1 < q) s1a -= S[q * TP + k + 1] * x[k + 1];
+        }
+        x[q] = (s0a + s1a) / S[q * TP + q];
     }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) inv[q * NB + i] = (q < w && i < w) ? x[q] : T(0);
 }
 
 // Panel: A_IJ <- A_IJ * inv^T for every block row I > J (one WG per I).
@@ -215,7 +234,11 @@ __global__ __launch_bounds__(256) void trsm_panel(T *a, Part P, int J, const T *
     const int r0 = P.start(I), rn = P.width(I), c0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, r0, c0);
     load_tile<T, false>(a, t, rn, w, Pn);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) Iv[(idx / NB) * TP + idx % NB] = inv[idx];
+#pragma unroll
+    for (int e = 0; e < NB * NB / 256; ++e) {
+        const int idx = threadIdx.x + 256 * e;
+        Iv[(idx / NB) * TP + idx % NB] = inv[idx];
+    }
     __syncthreads();
     // out[r][c] = sum_q Pn[r][q] * Iv[c][q]   (q <= c)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -281,10 +304,8 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
                 const int x = wr * 32 + i * 16 + mfma16_row<T>(lane, v);
                 const int y = wc * 32 + j * 16 + (lane & 15);
                 const int row = trans ? y : x, col = trans ? x : y;
-                T c = T(0);
-                if (row < rni && col < rnk && (!diagt || row >= col))
-                    c = a[tc.base + (size_t)row * tc.rs + (size_t)col * tc.cs];
-                acc[i][j][v] = c;
+                const T c = a[tc.base + (size_t)min(row, rni - 1) * tc.rs + (size_t)min(col, rnk - 1) * tc.cs];
+                acc[i][j][v] = (row < rni && col < rnk && (!diagt || row >= col)) ? c : T(0);
             }
     // panel tiles: thread owns 16 elements of each (coalesced along the contiguous axis)
     T pi[16], pk[16];
@@ -296,9 +317,11 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
             const int idx = threadIdx.x + 256 * e;
             int rr, cc;
             if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            pi[e] = (rr < rni && cc < w) ? a[ti.base + (size_t)rr * ti.rs + (size_t)cc * ti.cs] : T(0);
+            const T vi = a[ti.base + (size_t)min(rr, rni - 1) * ti.rs + (size_t)min(cc, w - 1) * ti.cs];
+            pi[e] = (rr < rni && cc < w) ? vi : T(0);
             if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            pk[e] = (rr < rnk && cc < w) ? a[tk.base + (size_t)rr * tk.rs + (size_t)cc * tk.cs] : T(0);
+            const T vk = a[tk.base + (size_t)min(rr, rnk - 1) * tk.rs + (size_t)min(cc, w - 1) * tk.cs];
+            pk[e] = (rr < rnk && cc < w) ? vk : T(0);
         }
     };
     auto sstore = [&](int J) {
@@ -391,8 +414,8 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
     const size_t s = (P.n & 1) ? P.n : P.n + 1;
     typedef typename Vec4<T>::type V4;
     V4 acc[4][4];
-    {
-        const Tile tc = tile_at(P, P.start(active ? I : 0), P.start(active ? K : 0));
+    if (active) {  // unconditional loads of clamped elements, then select
+        const Tile tc = tile_at(P, P.start(I), P.start(K));
         const T *cb = a + tc.base;
         const int trs = (int)tc.rs, tcs = (int)tc.cs;
 #pragma unroll
@@ -403,10 +426,14 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
                 for (int v = 0; v < 4; ++v) {
                     const int x = i * 16 + mfma16_row<T>(lane, v), y = j * 16 + (lane & 15);
                     const int row = trans ? y : x, col = trans ? x : y;
-                    T c = T(0);
-                    if (row < rni && col < rnk && (!diagt || row >= col)) c = cb[row * trs + col * tcs];
-                    acc[i][j][v] = c;
+                    const T c = cb[min(row, rni - 1) * trs + min(col, rnk - 1) * tcs];
+                    acc[i][j][v] = (row < rni && col < rnk && (!diagt || row >= col)) ? c : T(0);
                 }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = V4{0, 0, 0, 0};
     }
     // staging: slot b = 0,1 (I blocks), 2,3 (K blocks), 4 elements per thread each.
     // Column-major region: element (r, c) at k + 1 + r0 + r + c * s; row-major region:
@@ -677,21 +704,48 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     if (l2 != 0.0) {
         hipLaunchKernelGGL(add_diag<T>, dim3((n + 255) / 256), dim3(256), 0, stream, a, n, l2);
     }
-    for (int J0 = 0; J0 < P.nblk; J0 += NBO) {
-        const int J1 = J0 + NBO < P.nblk ? J0 + NBO : P.nblk;
+    // Look-ahead over outer panels of NBO blocks.  factor(p) = diagonal POTRF + inverse,
+    // panel TRSM and the in-panel updates of panel p; it runs on the aux stream while
+    // the caller's stream applies panel p-1 to everything right of panel p.  Per p:
+    //   caller: wait factor(p); update panel p+1's columns from p (64-wide tiles); mark N
+    //   aux:    wait N; factor(p+1); mark F
+    //   caller: update the columns right of panel p+1 from p (128-wide super-tiles)
+    // factor(p+1) touches only panel p+1's columns, disjoint from the concurrent update.
+    hipStream_t s2 = aux_stream();
+    hipEvent_t ev_start = aux_event(0), ev_f = aux_event(1), ev_n = aux_event(2);
+    KF_REQUIRE(s2 && ev_start && ev_f && ev_n, KF_ERR_HIP, "aux stream/event creation failed");
+    auto factor = [&](int J0, int J1) {
         for (int J = J0; J < J1; ++J) {
-            hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, stream, a, P, J, inv, info);
+            hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s2, a, P, J, inv, info);
             const int m = P.nblk - J - 1;
             if (m > 0)
-                hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, stream, a, P, J, inv, info);
+                hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, s2, a, P, J, inv, info);
             if (J + 1 < J1) {  // remaining columns of this outer panel
                 const long nt = range_tiles(P, J + 1, J1);
-                hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P,
-                                   J, J + 1, J + 1, J1, info);
+                hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, s2, a, P, J,
+                                   J + 1, J + 1, J1, info);
             }
         }
-        if (J1 < P.nblk) {  // everything right of the outer panel, 256-deep
-            const PairList L = make_pairs(P, J1);
+    };
+    KF_HIP_TRY(hipEventRecord(ev_start, stream));
+    KF_HIP_TRY(hipStreamWaitEvent(s2, ev_start, 0));
+    factor(0, NBO < P.nblk ? NBO : P.nblk);
+    KF_HIP_TRY(hipEventRecord(ev_f, s2));
+    for (int J0 = 0; J0 < P.nblk; J0 += NBO) {
+        const int J1 = J0 + NBO < P.nblk ? J0 + NBO : P.nblk;
+        const int J2 = J1 + NBO < P.nblk ? J1 + NBO : P.nblk;
+        KF_HIP_TRY(hipStreamWaitEvent(stream, ev_f, 0));
+        if (J1 < P.nblk) {
+            const long nt = range_tiles(P, J1, J2);
+            hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, J0,
+                               J1, J1, J2, info);
+            KF_HIP_TRY(hipEventRecord(ev_n, stream));
+            KF_HIP_TRY(hipStreamWaitEvent(s2, ev_n, 0));
+            factor(J1, J2);
+            KF_HIP_TRY(hipEventRecord(ev_f, s2));
+        }
+        if (J2 < P.nblk) {  // everything right of the next outer panel, NBO blocks deep
+            const PairList L = make_pairs(P, J2);
             const long nt = (long)L.np * (L.np + 1) / 2;
             hipLaunchKernelGGL(syrk2_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, L,
                                J0, J1, info);
