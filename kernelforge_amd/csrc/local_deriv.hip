@@ -18,10 +18,10 @@
 //    J onto difference vectors streamed over the features (v_m, u_m for the
 //    Hessian forms; s_i = sum_j c_m d_m and t_j = sum_i c_m d_m for the E-only
 //    Jacobian forms).  Writes EE / EF / FE, and e_m, v_m, u_m to a per-pair slot.
-//  * deriv_ff_kernel (FP64 MFMA 16x16x4): sum_i J_ai^T T_i with (feature, atom)
-//    chunks of J_a and J_b staged in LDS and the matching chunk of T formed in LDS
-//    from J_b (wave-uniform atom lists, lanes over columns) before the MFMA loop,
-//    then the rank-1 sum as a second MFMA pass over the slot; writes FF.
+//  * deriv_ff_kernel (FP64 MFMA 16x16x4): per feature f, T_f = C J_b[:,f,:] and
+//    FF += J_a[:,f,:]^T T_f, both on MFMA with operands streamed from HBM into
+//    registers (T_f's accumulators are the B operands of the second product), then
+//    the rank-1 sum as a second MFMA pass over the slot; writes FF.
 // Every output entry belongs to exactly one molecule pair, so no atomics are used
 // and results are deterministic.
 #include <algorithm>
@@ -41,8 +41,6 @@ constexpr int MAXA = 32;           // atoms per molecule supported here (3*MAXA 
 constexpr int MAXM = MAXA * MAXA;  // matched atom pairs per molecule pair
 constexpr int VCH = 12;            // difference vectors per projection pass
 constexpr int FCH = 128;           // features per staged projection chunk
-constexpr int MAXPB = 6;           // 16-column blocks per side (96 / 16)
-constexpr size_t FF_LDS_BYTES = 40 * 1024;  // J / T chunks; three work-groups fit one CU
 
 enum DerivMode {
     M_JAC = 0,         // (nm1, naq2)        EF
@@ -84,10 +82,6 @@ struct DerivArgs {
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// LDS row pitch (doubles) for n16 live columns.  Padding the pitch to 16 (mod 32) removes
-// the operand-read bank conflicts but costs a resident work-group per CU, which measured
-// slower (the kernel is latency-bound): rows stay unpadded.
-__host__ __device__ inline int lds_pitch(int n16) { return n16; }
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -399,129 +393,125 @@ __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) 
 }
 
 // ---------------------------------------------------------------------------
-// FF = sum_(f,i) J_a[i,f,:]^T T[(f,i),:]  -  sum_m w_m v_m u_m^T   on FP64 MFMA.
-// LDS: JA[fc][NI][NPp] | JB[fc][nb][NQp] | T[fc][NI][NQp] (dynamic), zero padded; per
-// feature chunk the work-group stages JA, JB, forms T from JB, then multiplies.
-// Wave w owns q-blocks {qb : qb == w mod 4} (all p-blocks) when there are at least
-// 3 q-blocks; with 1 or 2 q-blocks the waves also split the p-blocks.
+// FF = sum_f J_a[:,f,:]^T (C J_b[:,f,:])  -  sum_m w_m v_m u_m^T   on FP64 MFMA,
+// C[i][j] = c_ij for label-matched atoms (0 otherwise).  Per feature f both factors
+// are MFMA products:
+//   T_f = C x J_b[:,f,q-block]      (A = C from LDS, B = J_b rows straight from HBM)
+//   FF += J_a[:,f,:]^T x T_f        (A = J_a[:,f,:] staged in LDS, B = T_f)
+// The f64 16x16x4 accumulator layout (register v of lane l = row 4v + l/16, column
+// l%16) is exactly the B-operand layout of k-step v (lane l supplies row 4v + l/16,
+// column l%16), so T_f's accumulators feed the second product without a shuffle.
+// Wave w owns q-block w (16 force columns of b) and all p-blocks of a.  J_a[:,f,:] is
+// shared by the four waves: it is staged through two LDS buffers (one barrier per
+// feature); T for f+1 is formed while the product of f runs and the operands of f+2
+// are loaded during f.  Atoms are padded to 4*NK (rows of C past na / columns past nb
+// are zero, the matching Jacobian rows are clamped reads), force columns to 16*NP
+// (clamped reads, never stored).  Every global load is unconditional.
 // ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, int fc) {
-    extern __shared__ double lds[];
-    __shared__ int qa[MAXA], qb_[MAXA], ist[MAXA + 1], mj[MAXM];
-    __shared__ double mc[MAXM];
-    __shared__ int s_np;
+template <int MODE, int NK, int NP>
+__device__ __forceinline__ void ff_pair(const DerivArgs &A, int a, int b, int na, int nb,
+                                        const double *Cm, double *Jbuf, int np,
+                                        const double *mc, const double *slot) {
+    constexpr int NX = (NK + 3) / 4;         // 16-row fragments of C / T (atoms of a)
+    constexpr int FFR = 4 * NK, FFP = 16 * NP;  // staged J_a feature: rows x columns
+    constexpr int NEL = FFR * FFP, NST = (NEL + NT - 1) / NT;  // elements, per thread
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, col = lane & 15,
               kr = lane >> 4;
-    const int t = pair0 + pair_slot();
-    const int a = A.pairs[t].x, b = A.pairs[t].y;
-    const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
     const int rep = A.rep, nca = 3 * na, ncb = 3 * nb;
-    if (na == 0 || nb == 0) return;
-    const double *slot = A.slot + (size_t)(t - pair0) * A.slot_len;
-    if (tid < na) qa[tid] = A.q1[(size_t)a * A.ma1 + tid];
-    if (tid < nb) qb_[tid] = A.q2[(size_t)b * A.ma2 + tid];
-    __syncthreads();
-    if (tid == 0) {
-        int cnt = 0;
-        for (int i = 0; i < na; ++i) {
-            ist[i] = cnt;
-            for (int j = 0; j < nb; ++j)
-                if (qb_[j] == qa[i]) mj[cnt++] = j;
-        }
-        ist[na] = cnt;
-        s_np = cnt;
-    }
-    __syncthreads();
-    const int np = s_np;
-    for (int m = tid; m < np; m += NT) mc[m] = slot[m] * A.inv_s2;
-
-    const int NI = (na + 3) & ~3, npb = (nca + 15) >> 4, nqb = (ncb + 15) >> 4;
-    const int NPp = lds_pitch(npb * 16), NQp = lds_pitch(nqb * 16);
-    // wave -> (q-blocks, p-block subset)
-    int qlist[2] = {-1, -1}, p0 = 0, pstep = 1;
-    if (nqb >= 3) {
-        qlist[0] = wv < nqb ? wv : -1;
-        qlist[1] = wv + 4 < nqb ? wv + 4 : -1;
-    } else {
-        const int split = 4 / nqb;  // 4 (nqb = 1) or 2 (nqb = 2)
-        qlist[0] = wv % nqb;
-        p0 = wv / nqb;
-        pstep = split;
-    }
-    v4d acc[MAXPB][2];
-#pragma unroll
-    for (int x = 0; x < MAXPB; ++x) acc[x][0] = acc[x][1] = (v4d){0.0, 0.0, 0.0, 0.0};
-
-    double *JA = lds, *JB = lds + (size_t)fc * NI * NPp;
-    double *TT = JB + (size_t)fc * nb * NQp;
-    const size_t lda = 3 * (size_t)A.ma1, ldb = 3 * (size_t)A.ma2;
+    const int nqb = (ncb + 15) >> 4;
+    const int lda = 3 * A.ma1, ldb = 3 * A.ma2;
     const double *dxa = A.dx1 + (size_t)a * A.ma1 * rep * lda;
     const double *dxb = A.dx2 + (size_t)b * A.ma2 * rep * ldb;
-
-    for (int f0 = 0; f0 < rep; f0 += fc) {
-        const int fn = min(fc, rep - f0);
-        __syncthreads();
-        // rows (f, i) of NPp columns: lanes over columns, waves over rows
-        for (int row = wv; row < fc * NI; row += 4) {
-            const int f = row / NI, i = row - f * NI;
-            const bool live = f < fn && i < na;
-            const double *src = dxa + ((size_t)i * rep + f0 + f) * lda;
-            for (int p = lane; p < NPp; p += 64)
-                JA[(size_t)row * NPp + p] = (live && p < nca) ? src[p] : 0.0;
-        }
-        for (int row = wv; row < fc * nb; row += 4) {
-            const int f = row / nb, j = row - f * nb;
-            const bool live = f < fn;
-            const double *src = dxb + ((size_t)j * rep + f0 + f) * ldb;
-            for (int q = lane; q < NQp; q += 64)
-                JB[(size_t)row * NQp + q] = (live && q < ncb) ? src[q] : 0.0;
-        }
-        __syncthreads();
-        // T[(f, i), q] = sum_{j matched to i} c_ij JB[f][j][q]: one row per wave at a time
-        // (the atom list is wave-uniform), lanes over columns (coalesced LDS reads)
-        for (int row = wv; row < fc * NI; row += 4) {
-            const int f = row / NI, i = row - f * NI;
-            const int mb = i < na ? ist[i] : 0, me = i < na ? ist[i + 1] : 0;
-            const double *jb = JB + (size_t)f * nb * NQp;
-            for (int q = lane; q < NQp; q += 64) {
-                // four independent (index, row) load chains in flight
-                double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-                int m = mb;
-                for (; m + 3 < me; m += 4) {
-                    const int o0 = mj[m] * NQp, o1 = mj[m + 1] * NQp, o2 = mj[m + 2] * NQp,
-                              o3 = mj[m + 3] * NQp;
-                    const double c0 = mc[m], c1 = mc[m + 1], c2 = mc[m + 2], c3 = mc[m + 3];
-                    t0 = fma(c0, jb[o0 + q], t0);
-                    t1 = fma(c1, jb[o1 + q], t1);
-                    t2 = fma(c2, jb[o2 + q], t2);
-                    t3 = fma(c3, jb[o3 + q], t3);
-                }
-                for (; m < me; ++m) t0 = fma(mc[m], jb[mj[m] * NQp + q], t0);
-                TT[(size_t)row * NQp + q] = (t0 + t1) + (t2 + t3);
-            }
-        }
-        __syncthreads();
-        const int ksteps = fn * NI / 4;
-        for (int s = 0; s < ksteps; ++s) {
-            const int kk = 4 * s, f = kk / NI, i = kk - f * NI + kr;
-            const double *arow = JA + ((size_t)f * NI + i) * NPp + col;
-            const double *trow = TT + ((size_t)f * NI + i) * NQp + col;
+    // staging map: element e = tid + NT*u of a feature -> row e / FFP, column e % FFP
+    int soff[NST];
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int qb = qlist[k];
-                if (qb < 0) continue;
-                const double bv = trow[qb * 16];
-#pragma unroll
-                for (int x = 0; x < MAXPB; ++x) {
-                    const int pb = p0 + x * pstep;
-                    if (pb < npb)
-                        acc[x][k] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[pb * 16], bv,
-                                                                         acc[x][k], 0, 0, 0);
-                }
-            }
-        }
+    for (int u = 0; u < NST; ++u) {
+        const int e = min(tid + NT * u, NEL - 1), r = e / FFP, c = e % FFP;
+        soff[u] = min(r, na - 1) * rep * lda + min(c, nca - 1);
     }
+    // column-operand rows 4s + kr of b: affine in s except the last k-step (clamped)
+    const int sb = 4 * rep * ldb, rb0 = kr * rep * ldb;
+    const int rbL = min(4 * (NK - 1) + kr, nb - 1) * rep * ldb;
+    Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+    // rounds of four q-blocks (more than one only past 21 atoms of b)
+    for (int qr = 0; 4 * qr < nqb; ++qr) {
+    const int qb = 4 * qr + wv;
+    const bool busy = qb < nqb;
+    const int q = min(16 * qb + col, ncb - 1);
+
+    v4d acc[NP];
+#pragma unroll
+    for (int y = 0; y < NP; ++y) acc[y] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double st[NST], jb[2][NK];
+    v4d tt[2][NX];
+#define KF_FF_STAGE_LOAD(F)                                                                   \
+    do {                                                                                      \
+        const double *pa_ = dxa + (size_t)(F) * lda;                                          \
+        _Pragma("unroll") for (int u = 0; u < NST; ++u) st[u] = pa_[soff[u]];                 \
+    } while (0)
+#define KF_FF_STAGE_STORE(BUF)                                                                \
+    do {                                                                                      \
+        _Pragma("unroll") for (int u = 0; u < NST; ++u)                                       \
+            if (NEL % NT == 0 || tid + NT * u < NEL) (BUF)[tid + NT * u] = st[u];             \
+    } while (0)
+#define KF_FF_LOAD_B(S, F)                                                                    \
+    do {                                                                                      \
+        const double *pb_ = dxb + (size_t)(F) * ldb + q;                                      \
+        _Pragma("unroll") for (int s = 0; s < NK; ++s)                                        \
+            jb[S][s] = pb_[s < NK - 1 ? rb0 + s * sb : rbL];                                  \
+    } while (0)
+#define KF_FF_T(S)                                                                            \
+    do {                                                                                      \
+        _Pragma("unroll") for (int x = 0; x < NX; ++x) tt[S][x] = (v4d){0.0, 0.0, 0.0, 0.0};  \
+        _Pragma("unroll") for (int s = 0; s < NK; ++s)                                        \
+            _Pragma("unroll") for (int x = 0; x < NX; ++x)                                    \
+                tt[S][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(                              \
+                    Cm[(16 * x + col) * 33 + 4 * s + kr], jb[S][s], tt[S][x], 0, 0, 0);       \
+    } while (0)
+#define KF_FF_MAIN(S, BUF)                                                                    \
+    do {                                                                                      \
+        _Pragma("unroll") for (int s = 0; s < NK; ++s)                                        \
+            _Pragma("unroll") for (int y = 0; y < NP; ++y)                                    \
+                acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(                                \
+                    (BUF)[(4 * s + kr) * FFP + 16 * y + col], tt[S][s >> 2][s & 3], acc[y],   \
+                    0, 0, 0);                                                                 \
+    } while (0)
+    // prologue: J_a(0) in buffer 0, J_a(1) in registers, T(0), J_b(1)
+    double *const J0 = Jbuf, *const J1 = Jbuf + NEL;
+    KF_FF_STAGE_LOAD(0);
+    KF_FF_STAGE_STORE(J0);
+    KF_FF_STAGE_LOAD(min(1, rep - 1));
+    KF_FF_LOAD_B(0, 0);
+    KF_FF_T(0);
+    KF_FF_LOAD_B(1, min(1, rep - 1));
+    __syncthreads();
+    for (int f = 0; f < rep; f += 2) {
+        // feature f: buffer J0, T slot 0; prepare f+1 (J1, T slot 1) and loads for f+2
+        KF_FF_STAGE_STORE(J1);
+        KF_FF_STAGE_LOAD(min(f + 2, rep - 1));
+        if (busy) {
+            KF_FF_T(1);
+            KF_FF_MAIN(0, J0);
+        }
+        KF_FF_LOAD_B(0, min(f + 2, rep - 1));
+        __syncthreads();
+        if (f + 1 >= rep) break;
+        // feature f+1: buffer J1, T slot 1; prepare f+2 (J0, T slot 0) and loads for f+3
+        KF_FF_STAGE_STORE(J0);
+        KF_FF_STAGE_LOAD(min(f + 3, rep - 1));
+        if (busy) {
+            KF_FF_T(0);
+            KF_FF_MAIN(1, J1);
+        }
+        KF_FF_LOAD_B(1, min(f + 3, rep - 1));
+        __syncthreads();
+    }
+#undef KF_FF_STAGE_LOAD
+#undef KF_FF_STAGE_STORE
+#undef KF_FF_LOAD_B
+#undef KF_FF_T
+#undef KF_FF_MAIN
+    if (!busy) continue;
     // rank-1 part: acc -= sum_m (w_m v_m) u_m^T, K = matched pairs
     {
         const double *V = slot + A.maxm, *U = V + (size_t)A.maxm * A.ldv;
@@ -529,42 +519,86 @@ __global__ __launch_bounds__(NT) void deriv_ff_kernel(DerivArgs A, int pair0, in
             const int m = m0 + kr;
             const bool ok = m < np;
             const double w = ok ? -mc[m] * A.inv_s2 : 0.0;  // -e/s^4
+            const int qq = 16 * qb + col;
+            const double bv = (ok && qq < ncb) ? U[(size_t)m * A.ldu + qq] : 0.0;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int qb = qlist[k];
-                if (qb < 0) continue;
-                const int q = qb * 16 + col;
-                const double bv = (ok && q < ncb) ? U[(size_t)m * A.ldu + q] : 0.0;
-#pragma unroll
-                for (int x = 0; x < MAXPB; ++x) {
-                    const int pb = p0 + x * pstep;
-                    if (pb < npb) {
-                        const int p = pb * 16 + col;
-                        const double av = (ok && p < nca) ? w * V[(size_t)m * A.ldv + p] : 0.0;
-                        acc[x][k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[x][k], 0,
-                                                                         0, 0);
-                    }
-                }
+            for (int y = 0; y < NP; ++y) {
+                const int p = 16 * y + col;
+                const double av = (ok && p < nca) ? w * V[(size_t)m * A.ldv + p] : 0.0;
+                acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[y], 0, 0, 0);
             }
         }
     }
-    Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+    const int tc = 16 * qb + col;
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int qb = qlist[k];
-        if (qb < 0) continue;
-        const int tc = qb * 16 + col;
+    for (int y = 0; y < NP; ++y)
 #pragma unroll
-        for (int x = 0; x < MAXPB; ++x) {
-            const int pb = p0 + x * pstep;
-            if (pb >= npb) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = pb * 16 + mfma16_row_f64(lane, v);
-                if (r < nca && tc < ncb) out.ff(r, tc, acc[x][k][v]);
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * y + mfma16_row_f64(lane, v);
+            if (r < nca && tc < ncb) out.ff(r, tc, acc[y][v]);
         }
     }
+}
+
+// Shape buckets by m = max(atoms of a, atoms of b): (NK, NP) and the LDS they stage.
+inline void ff_bucket(int m, int &nk, int &np) {
+    if (m <= 5) { nk = 2; np = 1; }
+    else if (m <= 10) { nk = 3; np = 2; }
+    else if (m <= 16) { nk = 4; np = 3; }
+    else if (m <= 21) { nk = 6; np = 4; }
+    else if (m <= 24) { nk = 6; np = 5; }
+    else { nk = 8; np = 6; }
+}
+inline size_t ff_lds_bytes(int ma1, int ma2) {
+    int nk, np;
+    ff_bucket(ma1 > ma2 ? ma1 : ma2, nk, np);
+    return 2 * sizeof(double) * (size_t)(4 * nk) * (16 * np);
+}
+
+// Shape buckets by m = max(atoms of a, atoms of b): NK k-steps of 4 atoms, NP 16-column
+// force blocks (3 m columns).
+template <int MODE>
+__global__ __launch_bounds__(NT, 2) void deriv_ff_kernel(DerivArgs A, int pair0) {
+    __shared__ double Cm[32 * 33];
+    extern __shared__ double Jbuf[];  // two staged J_a features of the largest bucket
+    __shared__ int qa[MAXA], qb_[MAXA], ist[MAXA + 1], mi[MAXM], mj[MAXM];
+    __shared__ double mc[MAXM];
+    __shared__ int s_np;
+    const int tid = threadIdx.x;
+    const int t = pair0 + pair_slot();
+    const int a = A.pairs[t].x, b = A.pairs[t].y;
+    const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
+    if (na == 0 || nb == 0) return;
+    const double *slot = A.slot + (size_t)(t - pair0) * A.slot_len;
+    if (tid < na) qa[tid] = A.q1[(size_t)a * A.ma1 + tid];
+    if (tid < nb) qb_[tid] = A.q2[(size_t)b * A.ma2 + tid];
+    for (int e = tid; e < 32 * 33; e += NT) Cm[e] = 0.0;
+    __syncthreads();
+    if (tid == 0) {  // matched list in the prep kernel's (i-major) order
+        int cnt = 0;
+        for (int i = 0; i < na; ++i) {
+            ist[i] = cnt;
+            for (int j = 0; j < nb; ++j)
+                if (qb_[j] == qa[i]) { mi[cnt] = i; mj[cnt] = j; ++cnt; }
+        }
+        ist[na] = cnt;
+        s_np = cnt;
+    }
+    __syncthreads();
+    const int np = s_np;
+    for (int m = tid; m < np; m += NT) {
+        const double c = slot[m] * A.inv_s2;
+        mc[m] = c;
+        Cm[mi[m] * 33 + mj[m]] = c;
+    }
+    __syncthreads();
+    const int mx = na > nb ? na : nb;
+    if (mx <= 5) ff_pair<MODE, 2, 1>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
+    else if (mx <= 10) ff_pair<MODE, 3, 2>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
+    else if (mx <= 16) ff_pair<MODE, 4, 3>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
+    else if (mx <= 21) ff_pair<MODE, 6, 4>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
+    else if (mx <= 24) ff_pair<MODE, 6, 5>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
+    else ff_pair<MODE, 8, 6>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -619,7 +653,6 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
     A.pairs = reinterpret_cast<const int2 *>(dev + pair_off);
 
     size_t batch = npairs;
-    int fc = 1;
     if (has_ff(MODE)) {
         // slot layout and the largest matched count over the pairs
         int maxm = 1;
@@ -638,25 +671,16 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
         A.slot = static_cast<double *>(
             workspace(WS_AUX2, batch * A.slot_len * sizeof(double), stream));
         KF_REQUIRE(A.slot, KF_ERR_NOMEM, "scratch allocation failed");
-        const int NI = (A.ma1 + 3) & ~3;
-        const int pp = lds_pitch((3 * A.ma1 + 15) / 16 * 16), pq = lds_pitch((3 * A.ma2 + 15) / 16 * 16);
-        const int per_f = NI * pp + A.ma2 * pq + NI * pq;
-        fc = std::max(1, std::min(8, int(FF_LDS_BYTES / sizeof(double) / per_f)));
-        fc = std::min(fc, A.rep);
-        // keep fc * NI a multiple of 4 k-steps: NI already is
     } else {
         batch = std::min<size_t>(npairs, size_t(1) << 20);
     }
-    const int NI = (A.ma1 + 3) & ~3;
-    const int pp = lds_pitch((3 * A.ma1 + 15) / 16 * 16), pq = lds_pitch((3 * A.ma2 + 15) / 16 * 16);
-    const size_t lds_ff = sizeof(double) * (size_t)fc * (NI * pp + A.ma2 * pq + NI * pq);
     for (size_t p0 = 0; p0 < npairs; p0 += batch) {
         const unsigned nb = (unsigned)std::min(batch, npairs - p0);
         hipLaunchKernelGGL(deriv_prep_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0);
         KF_HIP_TRY(hipGetLastError());
         if (has_ff(MODE)) {
-            hipLaunchKernelGGL(deriv_ff_kernel<MODE>, dim3(nb), dim3(NT), lds_ff, stream, A,
-                               (int)p0, fc);
+            hipLaunchKernelGGL(deriv_ff_kernel<MODE>, dim3(nb), dim3(NT), ff_lds_bytes(A.ma1, A.ma2),
+                               stream, A, (int)p0);
             KF_HIP_TRY(hipGetLastError());
         }
     }
