@@ -393,6 +393,183 @@ __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) 
 }
 
 // ---------------------------------------------------------------------------
+// Projections for the Hessian forms on FP64 MFMA, pairs grouped by a shared molecule.
+//   v_m = J_ai^T d_m,  u_m = J_bj^T d_m,  d_m = x_ai - x_bj   (63-vectors for 21 atoms)
+// u of pair (a, b) at (i, j) is -v of the ordered pair (b, a) at (j, i), so one kernel
+// computes v for ordered pairs (x, y): a group is a molecule x and up to PG partners
+// y_k.  Side 0 groups the stored pairs (x, y) by their first molecule and writes V (and
+// e_m); side 1 groups them by their second molecule (x = b, y = a) and writes U = -v.
+// For one atom i of x the columns are every label-matched (k, j) of the group:
+//   V_i (3 n_x x ncol) = J_xi^T (3 n_x x rep) * D (rep x ncol),  D[f][c] = x_xi[f] - x_yj[f]
+// which streams each J_xi row once for the whole group (the one-pair form streamed it
+// once per pair).  Wave w takes column fragments w, w+4, ...; A operands (J rows) and B
+// operands (differences, formed in registers) come straight from HBM, 16 features ahead.
+// ---------------------------------------------------------------------------
+constexpr int PG = 8;  // partners per group
+
+struct ProjGroup {
+    int x, n, side, pad;
+    int y[PG];     // partner molecules
+    int slot[PG];  // batch-local slot of the stored pair
+};
+
+template <int NPF>
+__global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const ProjGroup *groups) {
+    __shared__ int qx[MAXA], qy[PG][MAXA], ny[PG], ist[PG][MAXA], rnk[PG][MAXA], rx[MAXA];
+    __shared__ int cols[PG * MAXA], wbase[5], gy[PG], gslot[PG];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, col = lane & 15, kr = lane >> 4;
+    const ProjGroup *gp = groups + blockIdx.x;
+    const bool s0 = gp->side == 0;
+    // side 0: x from set 1, partners from set 2; side 1 the other way round
+    const double *Xx = s0 ? A.x1 : A.x2, *Xy = s0 ? A.x2 : A.x1;
+    const double *Jx = s0 ? A.dx1 : A.dx2;
+    const int *Qx = s0 ? A.q1 : A.q2, *Qy = s0 ? A.q2 : A.q1;
+    const int *Nx = s0 ? A.n1 : A.n2, *Ny = s0 ? A.n2 : A.n1;
+    const int max_ = s0 ? A.ma1 : A.ma2, may = s0 ? A.ma2 : A.ma1;
+    const int rep = A.rep, x = gp->x, n = gp->n;
+    const int nx = clampi(Nx[x], 0, max_), ncx = 3 * nx;
+    const int lda = 3 * max_;
+    if (tid < MAXA) qx[tid] = tid < nx ? Qx[(size_t)x * max_ + tid] : -1;
+    if (tid < PG) {
+        gy[tid] = tid < n ? gp->y[tid] : 0;
+        gslot[tid] = tid < n ? gp->slot[tid] : 0;
+        ny[tid] = tid < n ? clampi(Ny[gp->y[tid]], 0, may) : 0;
+    }
+    __syncthreads();
+    {
+        const int k = tid >> 5, j = tid & 31;  // 8 x 32 = NT
+        qy[k][j] = (k < n && j < ny[k]) ? Qy[(size_t)gy[k] * may + j] : -2;
+    }
+    __syncthreads();
+    {
+        // m-offset tables.  Side 0, pair (x, y): m(i, j) = ist[k][i] + rnk[k][j], with
+        // ist = sum over atoms i' < i of x of the partner's count of label q(i') and rnk =
+        // rank of j among the partner's atoms of its label.  Side 1, pair (y, x):
+        // m(j, i) = ist[k][j] + rx[i], ist summing x's counts over the partner's atoms j' < j.
+        const int k = tid >> 5, t = tid & 31;
+        if (k < n) {
+            int s = 0;
+            if (s0) {
+                for (int i2 = 0; i2 < t && i2 < nx; ++i2)
+                    for (int j2 = 0; j2 < ny[k]; ++j2) s += qy[k][j2] == qx[i2];
+            } else {
+                for (int j2 = 0; j2 < t && j2 < ny[k]; ++j2)
+                    for (int i2 = 0; i2 < nx; ++i2) s += qx[i2] == qy[k][j2];
+            }
+            ist[k][t] = s;
+            int r = 0;
+            for (int j2 = 0; j2 < t; ++j2) r += qy[k][j2] == qy[k][t];
+            rnk[k][t] = r;
+        }
+        if (tid < MAXA) {
+            int r = 0;
+            for (int i2 = 0; i2 < tid; ++i2) r += qx[i2] == qx[tid];
+            rx[tid] = r;
+        }
+    }
+    const double *xx0 = Xx + (size_t)x * max_ * rep;
+    const double *Jx0 = Jx + (size_t)x * max_ * rep * lda;
+    int pofs[NPF];
+#pragma unroll
+    for (int y = 0; y < NPF; ++y) pofs[y] = min(16 * y + col, max(ncx, 1) - 1);
+
+    for (int i = 0; i < nx; ++i) {
+        __syncthreads();  // tables ready / the previous atom's columns consumed
+        const int l = qx[i];
+        {
+            // columns: every (k, j) with q_y[k][j] == l, k-major
+            const int k = tid >> 5, j = tid & 31;
+            const bool v = qy[k][j] == l;
+            const unsigned long long bal = __ballot(v);
+            const int pre = __popcll(bal & ((1ull << lane) - 1));
+            if (lane == 0) wbase[wv + 1] = __popcll(bal);
+            __syncthreads();
+            if (tid == 0) {
+                wbase[0] = 0;
+                for (int w = 1; w <= 4; ++w) wbase[w] += wbase[w - 1];
+            }
+            __syncthreads();
+            if (v) cols[wbase[wv] + pre] = (k << 5) | j;
+            __syncthreads();
+        }
+        const int ncol = wbase[4];
+        const int ncf = (ncol + 15) >> 4;
+        const double *xi = xx0 + (size_t)i * rep;
+        const double *Ji = Jx0 + (size_t)i * rep * lda;
+        for (int z = wv; z < ncf; z += 4) {
+            const int c = min(16 * z + col, ncol - 1);
+            const int kc = cols[c] >> 5, jc = cols[c] & 31;
+            const double *xj = Xy + ((size_t)gy[kc] * may + jc) * rep;
+            v4d acc[NPF];
+#pragma unroll
+            for (int y = 0; y < NPF; ++y) acc[y] = (v4d){0.0, 0.0, 0.0, 0.0};
+            double d2 = 0.0;
+            // 16 features (4 k-steps) per block; the next block is loaded during this one
+            double ja[2][4][NPF], da[2][4], db[2][4];
+#define KF_VP_LOAD(S, F0)                                                                      \
+    do {                                                                                       \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                        \
+            const int f_ = min((F0) + 4 * u + kr, rep - 1);                                    \
+            da[S][u] = xi[f_];                                                                 \
+            db[S][u] = xj[f_];                                                                 \
+            _Pragma("unroll") for (int y = 0; y < NPF; ++y)                                    \
+                ja[S][u][y] = Ji[(size_t)f_ * lda + pofs[y]];                                  \
+        }                                                                                      \
+    } while (0)
+#define KF_VP_MMA(S, F0)                                                                       \
+    do {                                                                                       \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                        \
+            const double d_ = (F0) + 4 * u + kr < rep ? da[S][u] - db[S][u] : 0.0;             \
+            d2 = fma(d_, d_, d2);                                                              \
+            _Pragma("unroll") for (int y = 0; y < NPF; ++y)                                    \
+                acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ja[S][u][y], d_, acc[y], 0, 0, 0); \
+        }                                                                                      \
+    } while (0)
+            KF_VP_LOAD(0, 0);
+            for (int f0 = 0; f0 < rep; f0 += 32) {
+                KF_VP_LOAD(1, f0 + 16);
+                KF_VP_MMA(0, f0);
+                if (f0 + 16 >= rep) break;
+                KF_VP_LOAD(0, f0 + 32);
+                KF_VP_MMA(1, f0 + 16);
+            }
+#undef KF_VP_LOAD
+#undef KF_VP_MMA
+            // |d|^2 of column c: the four lanes kr = 0..3 hold quarter sums
+            d2 += __shfl_xor(d2, 16, 64);
+            d2 += __shfl_xor(d2, 32, 64);
+            const int cc = 16 * z + col;
+            if (cc >= ncol) continue;
+            const int k = cols[cc] >> 5, j = cols[cc] & 31;
+            double *slot = A.slot + (size_t)gslot[k] * A.slot_len;
+            if (s0) {
+                const int m = ist[k][i] + rnk[k][j];
+                double *V = slot + A.maxm + (size_t)m * A.ldv;
+                if (kr == 0) slot[m] = exp(d2 * A.cexp);
+                // acc[y][v] is row p = 16y + 4v + kr of column cc
+#pragma unroll
+                for (int y = 0; y < NPF; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int p = 16 * y + 4 * v + kr;
+                        if (p < ncx) V[p] = acc[y][v];
+                    }
+            } else {
+                const int m = ist[k][j] + rx[i];
+                double *U = slot + A.maxm + (size_t)A.maxm * A.ldv + (size_t)m * A.ldu;
+#pragma unroll
+                for (int y = 0; y < NPF; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int p = 16 * y + 4 * v + kr;
+                        if (p < ncx) U[p] = -acc[y][v];
+                    }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FF = sum_f J_a[:,f,:]^T (C J_b[:,f,:])  -  sum_m w_m v_m u_m^T   on FP64 MFMA,
 // C[i][j] = c_ij for label-matched atoms (0 otherwise).  Per feature f both factors
 // are MFMA products:
@@ -568,13 +745,13 @@ __global__ __launch_bounds__(NT, 2) void deriv_ff_kernel(DerivArgs A, int pair0)
     const int t = pair0 + pair_slot();
     const int a = A.pairs[t].x, b = A.pairs[t].y;
     const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
-    if (na == 0 || nb == 0) return;
+    // a molecule without atoms still gets its zero EE / FE / EF entries written
     const double *slot = A.slot + (size_t)(t - pair0) * A.slot_len;
     if (tid < na) qa[tid] = A.q1[(size_t)a * A.ma1 + tid];
     if (tid < nb) qb_[tid] = A.q2[(size_t)b * A.ma2 + tid];
     for (int e = tid; e < 32 * 33; e += NT) Cm[e] = 0.0;
     __syncthreads();
-    if (tid == 0) {  // matched list in the prep kernel's (i-major) order
+    if (tid == 0) {  // matched list, i-major: the order of the slot entries
         int cnt = 0;
         for (int i = 0; i < na; ++i) {
             ist[i] = cnt;
@@ -592,6 +769,27 @@ __global__ __launch_bounds__(NT, 2) void deriv_ff_kernel(DerivArgs A, int pair0)
         Cm[mi[m] * 33 + mj[m]] = c;
     }
     __syncthreads();
+    {
+        // EE = sum_m e_m, FE = -sum_m c_m v_m, EF = sum_m c_m u_m from the slot
+        Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+        const double *V = slot + A.maxm, *U = V + (size_t)A.maxm * A.ldv;
+        if (has_ee(MODE) && tid == 0) {
+            double e = 0.0;
+            for (int m = 0; m < np; ++m) e += slot[m];
+            out.ee(e);
+        }
+        if (has_fe(MODE) && tid < 3 * na) {
+            double sfe = 0.0;
+            for (int m = 0; m < np; ++m) sfe -= mc[m] * V[(size_t)m * A.ldv + tid];
+            out.fe(tid, sfe);
+        }
+        if (has_ef(MODE) && tid >= 128 && tid - 128 < 3 * nb) {
+            double sef = 0.0;
+            for (int m = 0; m < np; ++m) sef += mc[m] * U[(size_t)m * A.ldu + tid - 128];
+            out.ef(tid - 128, sef);
+        }
+    }
+    if (na == 0 || nb == 0) return;
     const int mx = na > nb ? na : nb;
     if (mx <= 5) ff_pair<MODE, 2, 1>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
     else if (mx <= 10) ff_pair<MODE, 3, 2>(A, a, b, na, nb, Cm, Jbuf, np, mc, slot);
@@ -674,11 +872,74 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
     } else {
         batch = std::min<size_t>(npairs, size_t(1) << 20);
     }
-    for (size_t p0 = 0; p0 < npairs; p0 += batch) {
+    // Hessian forms: the projection groups of every batch, built once and uploaded once
+    // (side 0: runs of stored pairs with the same first molecule; side 1: the same pairs
+    // grouped by their second molecule)
+    std::vector<size_t> gstart;
+    const ProjGroup *dgroups = nullptr;
+    if (has_ff(MODE)) {
+        std::vector<ProjGroup> groups;
+        std::vector<std::pair<int, int>> byb;
+        for (size_t p0 = 0; p0 < npairs; p0 += batch) {
+            const int nbat = (int)std::min(batch, npairs - p0);
+            gstart.push_back(groups.size());
+            for (int k = 0; k < nbat;) {
+                ProjGroup g{};
+                g.x = pairs[p0 + k].x;
+                g.side = 0;
+                while (k < nbat && pairs[p0 + k].x == g.x && g.n < PG) {
+                    g.y[g.n] = pairs[p0 + k].y;
+                    g.slot[g.n++] = k++;
+                }
+                groups.push_back(g);
+            }
+            byb.clear();
+            for (int k = 0; k < nbat; ++k) byb.emplace_back(pairs[p0 + k].y, k);
+            std::stable_sort(byb.begin(), byb.end(),
+                             [](const std::pair<int, int> &u, const std::pair<int, int> &v) {
+                                 return u.first < v.first;
+                             });
+            for (size_t k = 0; k < byb.size();) {
+                ProjGroup g{};
+                g.x = byb[k].first;
+                g.side = 1;
+                while (k < byb.size() && byb[k].first == g.x && g.n < PG) {
+                    g.y[g.n] = pairs[p0 + byb[k].second].x;
+                    g.slot[g.n++] = byb[k].second;
+                    ++k;
+                }
+                groups.push_back(g);
+            }
+        }
+        gstart.push_back(groups.size());
+        const size_t gb = sizeof(ProjGroup) * groups.size();
+        ProjGroup *dg = static_cast<ProjGroup *>(workspace(WS_AUX3, gb, stream));
+        ProjGroup *hg = static_cast<ProjGroup *>(host_staging(WS_AUX3, gb));
+        KF_REQUIRE(dg && hg, KF_ERR_NOMEM, "workspace allocation failed (%zu B)", gb);
+        std::copy(groups.begin(), groups.end(), hg);
+        KF_HIP_TRY(hipMemcpyAsync(dg, hg, gb, hipMemcpyHostToDevice, stream));
+        dgroups = dg;
+    }
+    const int npf = (3 * std::max(A.ma1, A.ma2) + 15) / 16;
+    size_t bi = 0;
+    for (size_t p0 = 0; p0 < npairs; p0 += batch, ++bi) {
         const unsigned nb = (unsigned)std::min(batch, npairs - p0);
-        hipLaunchKernelGGL(deriv_prep_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0);
-        KF_HIP_TRY(hipGetLastError());
+        if (!has_ff(MODE)) {
+            hipLaunchKernelGGL(deriv_prep_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0);
+            KF_HIP_TRY(hipGetLastError());
+        }
         if (has_ff(MODE)) {
+            const unsigned ng = (unsigned)(gstart[bi + 1] - gstart[bi]);
+            const ProjGroup *gp = dgroups + gstart[bi];
+            switch (npf) {
+                case 1: hipLaunchKernelGGL(deriv_vproj_kernel<1>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+                case 2: hipLaunchKernelGGL(deriv_vproj_kernel<2>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+                case 3: hipLaunchKernelGGL(deriv_vproj_kernel<3>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+                case 4: hipLaunchKernelGGL(deriv_vproj_kernel<4>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+                case 5: hipLaunchKernelGGL(deriv_vproj_kernel<5>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+                default: hipLaunchKernelGGL(deriv_vproj_kernel<6>, dim3(ng), dim3(NT), 0, stream, A, gp); break;
+            }
+            KF_HIP_TRY(hipGetLastError());
             hipLaunchKernelGGL(deriv_ff_kernel<MODE>, dim3(nb), dim3(NT), ff_lds_bytes(A.ma1, A.ma2),
                                stream, A, (int)p0);
             KF_HIP_TRY(hipGetLastError());
