@@ -211,7 +211,7 @@ def ef_leg(args, L):
         "breakdown_ms": {k: (round(v, 3) if v is not None else None) for k, v in ms.items()},
         "roofline": {"bound": "mfma", "achieved": round(kt, 3), "peak": PEAK_F64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(kt / PEAK_F64_MFMA_TFLOPS, 4),
-                     "traffic": None, "kernel": "deriv_prep_kernel + deriv_ff_kernel "
+                     "traffic": None, "kernel": "deriv_vproj_kernel + deriv_ff_kernel "
                      "<FULL_SYMM_RFP>", "flop_per_launch": flop},
         "test_mae": {"energy_ha": round(e_mae, 6), "force_ha_per_a": round(f_mae, 6),
                      "n_test_conformers": ntest},
