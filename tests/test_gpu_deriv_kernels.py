@@ -33,11 +33,11 @@ def rand_set(nm, ma, rep, nlab, seed, nmin=0):
     return X, dX, Q, N
 
 
-# (nm1, ma1, nm2, ma2, rep, nlab): single-atom molecules (one 16-column block, the
-# waves split p-blocks), 2 q-blocks, the 4-wave q split, >64 columns (two projection
-# passes) and the 32-atom maximum.
+# (nm1, ma1, nm2, ma2, rep, nlab): single-atom molecules (one 16-column block), 2 and 4
+# force-column blocks, more than 64 columns (a second round of column blocks), the
+# 32-atom maximum, and more partners per molecule than one projection group holds (8).
 CASES = [(3, 1, 4, 2, 9, 1), (5, 3, 6, 4, 33, 2), (4, 7, 5, 9, 40, 3), (3, 21, 4, 21, 72, 3),
-         (2, 23, 3, 25, 50, 2), (2, 32, 2, 32, 24, 2)]
+         (2, 23, 3, 25, 50, 2), (2, 32, 2, 32, 24, 2), (3, 5, 20, 7, 16, 2)]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -60,7 +60,7 @@ def test_rect_forms(lk, case):
 
 
 @pytest.mark.parametrize("case", [(4, 2, 10, 2), (7, 6, 33, 3), (5, 21, 64, 3), (3, 24, 20, 2),
-                                  (2, 32, 12, 1)])
+                                  (2, 32, 12, 1), (19, 6, 20, 3)])
 def test_symm_forms(lk, case):
     nm, ma, rep, nlab = case
     X, dX, Q, N = rand_set(nm, ma, rep, nlab, 7 * ma + nm)
