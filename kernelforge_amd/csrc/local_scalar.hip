@@ -44,26 +44,133 @@ constexpr int NT = 256;      // threads per workgroup
 
 enum Mode { SYMM_RFP = 0, SYMM_DENSE = 1, RECT = 2 };
 
-// pack: Xp[row, k] = X[src[row], k] (k < rep), 0 for rep <= k < rep_pad;
-// n2[row] = |x|^2 in FP64.  One wave per row.
+// Column windows: the descriptor columns are cut into windows of KS = 16.  A window that is
+// zero in every row of a row tile, or in every row of the column tiles it meets, adds an
+// exact zero to every Gram entry of that tile pair, so the kernel leaves it out.  Masks of
+// the first kMaxWin windows are kept as two 64-bit words; wider descriptors run unmasked.
+constexpr int kMaxWin = 128;
+
+// Window bits of one 64-column slice from a wave ballot of "column is non-zero".
+__device__ __forceinline__ unsigned long long slice_windows(unsigned long long b) {
+    unsigned long long w = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if ((b >> (16 * q)) & 0xFFFFull) w |= 1ull << q;
+    return w;
+}
+
+// mask[a] = windows in which some atom of molecule a is non-zero (two words per molecule).
+// One wave per molecule.
+__global__ __launch_bounds__(256) void mol_winmask_kernel(const double *__restrict__ X,
+                                                          const int *__restrict__ N, int nm,
+                                                          int ma, int rep,
+                                                          unsigned long long *__restrict__ mask) {
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (a >= nm) return;
+    const int na = min(max(N[a], 0), ma);
+    const double *x = X + (size_t)a * ma * rep;
+    unsigned long long w0 = 0, w1 = 0;
+    for (int j = 0; j * 64 < rep && 4 * j < kMaxWin; ++j) {
+        const int k = j * 64 + lane;
+        bool nz = false;
+        if (k < rep)
+            for (int i = 0; i < na; ++i) nz |= x[(size_t)i * rep + k] != 0.0;
+        const unsigned long long w = slice_windows(__ballot(nz));
+        if (j < 16) w0 |= w << (4 * j);
+        else w1 |= w << (4 * j - 64);
+    }
+    if (lane == 0) {
+        mask[2 * (size_t)a] = w0;
+        mask[2 * (size_t)a + 1] = w1;
+    }
+}
+
+// pack: Xp[row, k] = X[xrow, k] (k < rep), 0 for rep <= k < rep_pad, where the plan's
+// source s = src[row] names molecule s / ma of the (possibly reordered) set and xrow =
+// perm[s / ma] * ma + s % ma its row of X (perm == nullptr: identity);
+// n2[row] = |x|^2 in FP64; rowmask[row] = the row's non-zero windows.  One wave per row.
 __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ X,
-                                                     const int *__restrict__ src, int rows,
-                                                     int rep, int rep_pad,
+                                                     const int *__restrict__ src,
+                                                     const int *__restrict__ perm, int ma,
+                                                     int rows, int rep, int rep_pad,
                                                      double *__restrict__ Xp,
-                                                     double *__restrict__ n2) {
+                                                     double *__restrict__ n2,
+                                                     unsigned long long *__restrict__ rowmask) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= rows) return;
-    const double *s = X + (size_t)src[row] * rep;
+    const int sr = src[row];
+    const size_t xrow = perm ? (size_t)perm[sr / ma] * ma + sr % ma : (size_t)sr;
+    const double *s = X + xrow * rep;
     double *d = Xp + (size_t)row * rep_pad;
     double acc = 0.0;
-    for (int k = lane; k < rep_pad; k += 64) {
+    unsigned long long w0 = 0, w1 = 0;
+    for (int j = 0; j * 64 < rep_pad; ++j) {
+        const int k = j * 64 + lane;
         const double v = k < rep ? s[k] : 0.0;
-        d[k] = v;
+        if (k < rep_pad) d[k] = v;
         acc += v * v;
+        const unsigned long long w = slice_windows(__ballot(v != 0.0));
+        if (j < 16) w0 |= w << (4 * j);
+        else if (j < kMaxWin / 4) w1 |= w << (4 * j - 64);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) n2[row] = acc;
+    if (lane == 0) {
+        n2[row] = acc;
+        rowmask[2 * (size_t)row] = w0;
+        rowmask[2 * (size_t)row + 1] = w1;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
+    unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+    for (int off = 32; off > 0; off >>= 1) {
+        lo |= (unsigned)__shfl_xor((int)lo, off);
+        hi |= (unsigned)__shfl_xor((int)hi, off);
+    }
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Window masks of the kernel's row tiles: for label l and molecule block A, the block's
+// rows of label l are cut into 64-row tiles from its first row (the tiling the kernel
+// walks); tmask[tbase[l * nblk + A] + t] = OR of the masks of tile t's rows.  One
+// work-group per (label, block), a wave per tile.
+__global__ __launch_bounds__(256) void tile_mask_kernel(const unsigned long long *__restrict__ rowmask,
+                                                        const int *__restrict__ mol_row, int nm,
+                                                        int nblk, const int *__restrict__ tbase,
+                                                        ulonglong2 *__restrict__ tmask) {
+    const int la = blockIdx.x, l = la / nblk, A = la - l * nblk;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int *mr = mol_row + (size_t)l * (nm + 1);
+    const int r0 = mr[A * MB], r1 = mr[min(A * MB + MB, nm)];
+    const int nt = (r1 - r0 + TS - 1) / TS;
+    for (int t = wave; t < nt; t += 4) {
+        const int r = r0 + t * TS + lane;
+        unsigned long long m0 = 0, m1 = 0;
+        if (r < r1) {
+            m0 = rowmask[2 * (size_t)r];
+            m1 = rowmask[2 * (size_t)r + 1];
+        }
+        m0 = wave_or64(m0);
+        m1 = wave_or64(m1);
+        if (lane == 0) tmask[tbase[la] + t] = make_ulonglong2(m0, m1);
+    }
+}
+
+// Position of the n-th (0-based) set bit of m; n < popcount(m).
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
+    int pos = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const int c = __popcll(m & ((1ull << s) - 1));
+        if (n >= c) {
+            n -= c;
+            m >>= s;
+            pos += s;
+        }
+    }
+    return pos;
 }
 
 
@@ -87,8 +194,19 @@ struct GramArgs {
     // rows and columns past a tile's end repeat a valid one and their Gram entries are
     // never read, so every load is unconditional (exact wait counts, no branches)
     const double *pa, *pb0, *pb1;
-    int rep_pad, lofs, r16, kq;
+    int lofs, r16, kq;
+    // the tile pair's active windows in ascending order (an even count, nsub): entry e
+    // is held by lane e % 64 of w0 (e < 64) or w1 (64 <= e < kMaxWin); entries past
+    // kMaxWin (unmasked wide descriptors only) are window e itself
+    int nsub, w0, w1;
 };
+
+// Column offset of active window e (e uniform across the wave).
+__device__ __forceinline__ int win_off(const GramArgs &g, int e) {
+    const int lo = __builtin_amdgcn_readlane(g.w0, e & 63);
+    const int hi = __builtin_amdgcn_readlane(g.w1, e & 63);
+    return (e < 64 ? lo : (e < kMaxWin ? hi : e)) * KS;
+}
 
 template <int NTL, int NA, int NTW>
 __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c)[2][4]) {
@@ -116,13 +234,17 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
             bq[slot][NB - 1][1] = *(const double2 *)(g.pb1 + (k) + 2);   \
         }                                                                \
     } while (0)
-    const int nch = g.rep_pad / KA, nsub = nch * NSUB;
+    const int nsub = g.nsub, nch = nsub / NSUB;
+    if (nch == 0) {  // no window is active: the Gram block is exactly zero
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < NSUB; ++s) {
-        KF_LOAD_A(s * KS);
+        KF_LOAD_A(win_off(g, s));
         KF_STORE_A(Abuf, s);
     }
-    if (NTW > 0) KF_LOAD_B(0, 0);
+    if (NTW > 0) KF_LOAD_B(0, win_off(g, 0));
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
         const double *cur = Abuf + (ch & 1) * (TS * LDA);
@@ -137,8 +259,8 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
             // In the last chunk the parts land in the idle buffer and are never read.
             __builtin_amdgcn_sched_barrier(0);
             if (s > 0) KF_STORE_A(nxt, s - 1);
-            KF_LOAD_A(cn * KA + s * KS);
-            if (NTW > 0) KF_LOAD_B((s + 1) % NSUB, min(gs + 1, nsub - 1) * KS);
+            KF_LOAD_A(win_off(g, cn * NSUB + s));
+            if (NTW > 0) KF_LOAD_B((s + 1) % NSUB, win_off(g, min(gs + 1, nsub - 1)));
             __builtin_amdgcn_sched_barrier(0);
             if (NTW == 0) continue;
 #pragma unroll
@@ -207,13 +329,23 @@ __device__ __forceinline__ double scan_step(double v, bool take) {
 // the segment tails go to LDS as partial sums P[row][segment], and each thread adds,
 // for its column molecule's segments, the rows of its row molecules.  The molecule of a
 // packed row is src[row] / max_atoms (src = a * max_atoms + i, see local_plan.hpp).
+// The molecules of each set may be reordered (molecules that use the same descriptor
+// windows sorted together, so that blocks share their zero windows): the kernel works in
+// that order and perm maps a molecule back to its row / column of the output.
+struct WinArgs {
+    const ulonglong2 *tm1, *tm2;  // tile window masks (null: every window active)
+    const int *tb1, *tb2;         // first tile mask of (label, block), l * nblk + block
+    const int *perm1, *perm2;     // output index of each molecule (null: identity)
+    int nblk1, nblk2, nwin;       // blocks per set, windows of the padded descriptor
+};
+
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
     const int *__restrict__ src1, int ma1, const double *__restrict__ Xp2,
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
     int ma2, int L, int nm1, int nm2, int rep_pad, double cexp, const int2 *__restrict__ order,
-    double *__restrict__ out) {
+    const WinArgs wa, double *__restrict__ out) {
     // two row-operand buffers; the segment partial sums of the epilogue reuse the same
     // memory once the Gram products are done
     constexpr int OPS = 2 * TS * LDA, EPI = TS * PP;
@@ -238,7 +370,6 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const int ob = tid & (MB - 1), oa0 = (tid / MB) * (MB / 4);
     const int lrow = tid >> 2, bcol = wave * 16 + (lane & 15);
     GramArgs ga;
-    ga.rep_pad = rep_pad;
     ga.lofs = lrow * LDA + (tid & 3) * 4;
     ga.r16 = lane & 15;
     ga.kq = lane >> 4;
@@ -251,6 +382,8 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
         __syncthreads();
         const int ra0 = mrA[0], ra1 = mrA[na], rb0 = mrB[0], rb1 = mrB[nb];
         if (ra1 == ra0 || rb1 == rb0) continue;
+        const int tbA = wa.tm1 ? wa.tb1[l * wa.nblk1 + A] : 0;
+        const int tbB = wa.tm1 ? wa.tb2[l * wa.nblk2 + B] : 0;
         const int nsr = (ra1 - ra0 + TS - 1) / TS, nsc = (rb1 - rb0 + TS - 1) / TS;
         for (int sr = 0; sr < nsr; ++sr) {
             for (int sc0 = diag ? sr : 0; sc0 < nsc; sc0 += 2) {
@@ -309,6 +442,31 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                 ga.pb0 = Xp2 + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad + 4 * ga.kq;
                 ga.pb1 = ntile == 2 ? Xp2 + (size_t)(c0[1] + min(bcol, cn[1] - 1)) * rep_pad + 4 * ga.kq
                                     : ga.pb0;
+                // active windows: non-zero in the row tile and in one of the column tiles
+                unsigned long long m0 = ~0ull, m1 = ~0ull;
+                int nsub = wa.nwin;
+                if (wa.tm1) {
+                    const ulonglong2 ta = wa.tm1[tbA + sr], tc0 = wa.tm2[tbB + sc0];
+                    const ulonglong2 tc1 = ntile == 2 ? wa.tm2[tbB + sc0 + 1] : make_ulonglong2(0, 0);
+                    m0 = ta.x & (tc0.x | tc1.x);
+                    m1 = ta.y & (tc0.y | tc1.y);
+                    nsub = __popcll(m0) + __popcll(m1);
+                    if (nsub & 1) {  // even count: add an inactive window (it adds exact zeros)
+                        const int nw = wa.nwin;
+                        const unsigned long long v0 = nw >= 64 ? ~0ull : (1ull << nw) - 1;
+                        const unsigned long long v1 = nw >= 128 ? ~0ull : (nw > 64 ? (1ull << (nw - 64)) - 1 : 0ull);
+                        const unsigned long long z0 = ~m0 & v0, z1 = ~m1 & v1;
+                        if (z0) m0 |= z0 & (~z0 + 1);
+                        else m1 |= z1 & (~z1 + 1);
+                        ++nsub;
+                    }
+                }
+                {
+                    const int n0 = __popcll(m0);
+                    ga.w0 = lane < n0 ? nth_set_bit(m0, lane) : 64 + nth_set_bit(m1, lane - n0);
+                    ga.w1 = 64 + nth_set_bit(m1, 64 + lane - n0);
+                    ga.nsub = nsub;
+                }
                 const int na16 = (rn + 15) >> 4;
                 const int ntw = (wave * 16 < cn[1]) ? 2 : (wave * 16 < cn[0] ? 1 : 0);
                 if (ntile == 2)
@@ -376,51 +534,165 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
         const int a = oa0 + k, b = ob;
         if (a >= na || b >= nb) continue;
         const double val = accL[a * AP + b];
+        const size_t ia = wa.perm1 ? wa.perm1[a0 + a] : a0 + a;
+        const size_t ib = wa.perm2 ? wa.perm2[b0 + b] : b0 + b;
         if (MODE == SYMM_RFP) {
-            if (!diag || a <= b) out[rfp_upper((size_t)nm1, (size_t)(a0 + a), (size_t)(b0 + b))] = val;
+            if (!diag || a <= b) out[rfp_upper((size_t)nm1, ia < ib ? ia : ib, ia < ib ? ib : ia)] = val;
         } else if (MODE == RECT) {
-            out[(size_t)(a0 + a) * nm2 + b0 + b] = val;
+            out[ia * nm2 + ib] = val;
         } else if (!diag || a <= b) {
-            out[(size_t)(a0 + a) * nm1 + b0 + b] = val;
-            out[(size_t)(b0 + b) * nm1 + a0 + a] = val;
+            out[ia * nm1 + ib] = val;
+            out[ib * nm1 + ia] = val;
         }
     }
 }
 
-// Device copy of a plan plus the packed rows.
+// Device copy of a plan plus the packed rows, their window masks and the row tiles'.
 struct DevicePacked {
     double *Xp = nullptr, *n2 = nullptr;
-    int *mol_row = nullptr, *src = nullptr;
-    int rows = 0, rep_pad = 0;
+    int *mol_row = nullptr, *src = nullptr, *perm = nullptr, *tbase = nullptr;
+    unsigned long long *rowmask = nullptr;
+    ulonglong2 *tmask = nullptr;  // null: windows not masked
+    int rows = 0, rep_pad = 0, nblk = 0;
 };
 
-int upload_and_pack(const LocalPlan &p, const double *X, int rep, int ws_pack, int ws_plan,
-                    hipStream_t stream, DevicePacked &d) {
+inline int padded_rep(int rep) { return (rep + KA - 1) / KA * KA; }
+inline bool windows_masked(int rep) { return padded_rep(rep) / KS <= kMaxWin; }
+
+// Window masks of every molecule of a set (device X) -> host (two words per molecule).
+int molecule_windows(const double *X, const int *N, int nm, int ma, int rep, int ws,
+                     hipStream_t stream, std::vector<unsigned long long> &out) {
+    out.assign(2 * (size_t)nm, 0ull);
+    if (nm == 0) return KF_OK;
+    const size_t bytes = 2 * sizeof(unsigned long long) * (size_t)nm;
+    auto *dm = static_cast<unsigned long long *>(workspace(ws, bytes, stream));
+    auto *hm = static_cast<unsigned long long *>(host_staging(ws, bytes));
+    KF_REQUIRE(dm && hm, KF_ERR_NOMEM, "workspace allocation failed (%zu B)", bytes);
+    hipLaunchKernelGGL(mol_winmask_kernel, dim3((nm + 3) / 4), dim3(256), 0, stream, X, N, nm, ma,
+                       rep, dm);
+    KF_HIP_TRY(hipGetLastError());
+    KF_HIP_TRY(hipMemcpyAsync(hm, dm, bytes, hipMemcpyDeviceToHost, stream));
+    KF_HIP_TRY(hipStreamSynchronize(stream));
+    std::copy(hm, hm + 2 * (size_t)nm, out.begin());
+    return KF_OK;
+}
+
+// Windows ranked rarest first by the number of molecules (of all given sets) using them.
+std::vector<int> rank_windows(const std::vector<const std::vector<unsigned long long> *> &sets,
+                              int nwin) {
+    std::vector<long> freq(nwin, 0);
+    for (const auto *m : sets)
+        for (size_t a = 0; 2 * a < m->size(); ++a)
+            for (int w = 0; w < nwin; ++w)
+                if (((*m)[2 * a + (w >> 6)] >> (w & 63)) & 1ull) ++freq[w];
+    std::vector<int> rank(nwin);
+    for (int w = 0; w < nwin; ++w) rank[w] = w;
+    std::stable_sort(rank.begin(), rank.end(), [&](int x, int y) { return freq[x] < freq[y]; });
+    return rank;
+}
+
+// Molecule order: ascending by the bit string of the molecule's windows read rarest
+// first (the 64 rarest windows), ties in input order.  Molecules that use the same
+// rare windows end up in the same blocks, so the other blocks' tiles skip them.
+std::vector<int> order_molecules(const std::vector<unsigned long long> &mask,
+                                 const std::vector<int> &rank) {
+    const int nm = (int)(mask.size() / 2), nk = std::min<int>(64, (int)rank.size());
+    std::vector<unsigned long long> key(nm, 0ull);
+    for (int a = 0; a < nm; ++a) {
+        unsigned long long k = 0;
+        for (int t = 0; t < nk; ++t) {
+            const int w = rank[t];
+            k = (k << 1) | ((mask[2 * (size_t)a + (w >> 6)] >> (w & 63)) & 1ull);
+        }
+        key[a] = k;
+    }
+    std::vector<int> perm(nm);
+    for (int a = 0; a < nm; ++a) perm[a] = a;
+    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+    return perm;
+}
+
+// Metadata of a set taken in the order perm (new molecule t = old molecule perm[t]).
+void permute_metadata(const std::vector<int> &Q, const std::vector<int> &N, int ma,
+                      const std::vector<int> &perm, std::vector<int> &Qp, std::vector<int> &Np) {
+    const size_t nm = perm.size();
+    Qp.resize(nm * ma);
+    Np.resize(nm);
+    for (size_t t = 0; t < nm; ++t) {
+        std::copy(Q.begin() + (size_t)perm[t] * ma, Q.begin() + (size_t)(perm[t] + 1) * ma,
+                  Qp.begin() + t * ma);
+        Np[t] = N[perm[t]];
+    }
+}
+
+// Upload the plan (plus the molecule order, if any), pack the rows and, when the windows
+// are masked, build the masks of the kernel's row tiles.
+int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const double *X, int rep,
+                    int ws_pack, int ws_plan, hipStream_t stream, DevicePacked &d) {
     d.rows = p.rows;
-    d.rep_pad = (rep + KA - 1) / KA * KA;
-    const size_t mr_n = p.mol_row.size(), src_n = p.src.size();
-    char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * (mr_n + src_n), stream));
-    const size_t pack_bytes = sizeof(double) * ((size_t)p.rows * d.rep_pad + p.rows + 64);
+    d.rep_pad = padded_rep(rep);
+    d.nblk = (p.nm + MB - 1) / MB;
+    const bool masked = windows_masked(rep);
+    const size_t mr_n = p.mol_row.size(), src_n = p.src.size(), perm_n = perm.size();
+    const size_t nla = (size_t)p.nlabels * d.nblk;
+    // tile-mask bases from the host plan: tiles of (label, block) in the kernel's tiling
+    std::vector<int> tbase(nla + 1, 0);
+    for (int l = 0; l < p.nlabels; ++l)
+        for (int A = 0; A < d.nblk; ++A) {
+            const int *mr = p.mol_row.data() + (size_t)l * (p.nm + 1);
+            const int rows = mr[std::min(A * MB + MB, p.nm)] - mr[A * MB];
+            tbase[(size_t)l * d.nblk + A + 1] = tbase[(size_t)l * d.nblk + A] + (rows + TS - 1) / TS;
+        }
+    const size_t ntile_masks = masked ? (size_t)tbase[nla] : 0;
+    const size_t plan_n = mr_n + src_n + perm_n + nla + 1;
+    char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * plan_n, stream));
+    const size_t xp_bytes = sizeof(double) * ((size_t)p.rows * d.rep_pad + p.rows + 64);
+    const size_t rm_off = (xp_bytes + 255) / 256 * 256;
+    const size_t tm_off = rm_off + (2 * sizeof(unsigned long long) * (size_t)p.rows + 255) / 256 * 256;
+    const size_t pack_bytes = tm_off + sizeof(ulonglong2) * (ntile_masks + 1);
     char *pack = static_cast<char *>(workspace(ws_pack, pack_bytes, stream));
     KF_REQUIRE(plan && pack, KF_ERR_NOMEM, "device workspace allocation failed (%zu B)",
                pack_bytes);
     d.mol_row = reinterpret_cast<int *>(plan);
-    int *src = d.mol_row + mr_n;
-    d.src = src;
-    int *stage = static_cast<int *>(host_staging(ws_plan, sizeof(int) * (mr_n + src_n)));
+    d.src = d.mol_row + mr_n;
+    d.perm = perm_n ? d.src + src_n : nullptr;
+    d.tbase = d.src + src_n + perm_n;
+    int *stage = static_cast<int *>(host_staging(ws_plan, sizeof(int) * plan_n));
     KF_REQUIRE(stage, KF_ERR_NOMEM, "pinned staging allocation failed");
     std::copy(p.mol_row.begin(), p.mol_row.end(), stage);
     std::copy(p.src.begin(), p.src.end(), stage + mr_n);
-    KF_HIP_TRY(hipMemcpyAsync(plan, stage, sizeof(int) * (mr_n + src_n), hipMemcpyHostToDevice,
-                              stream));
+    std::copy(perm.begin(), perm.end(), stage + mr_n + src_n);
+    std::copy(tbase.begin(), tbase.end(), stage + mr_n + src_n + perm_n);
+    KF_HIP_TRY(hipMemcpyAsync(plan, stage, sizeof(int) * plan_n, hipMemcpyHostToDevice, stream));
     d.Xp = reinterpret_cast<double *>(pack);
     d.n2 = d.Xp + (size_t)p.rows * d.rep_pad;
+    d.rowmask = reinterpret_cast<unsigned long long *>(pack + rm_off);
+    d.tmask = masked ? reinterpret_cast<ulonglong2 *>(pack + tm_off) : nullptr;
     if (p.rows > 0) {
-        hipLaunchKernelGGL(pack_rows_f64, dim3((p.rows + 3) / 4), dim3(256), 0, stream, X, src,
-                           p.rows, rep, d.rep_pad, d.Xp, d.n2);
+        hipLaunchKernelGGL(pack_rows_f64, dim3((p.rows + 3) / 4), dim3(256), 0, stream, X, d.src,
+                           d.perm, p.max_atoms, p.rows, rep, d.rep_pad, d.Xp, d.n2, d.rowmask);
         KF_HIP_TRY(hipGetLastError());
+        if (masked && nla > 0) {
+            hipLaunchKernelGGL(tile_mask_kernel, dim3((unsigned)nla), dim3(256), 0, stream,
+                               d.rowmask, d.mol_row, p.nm, d.nblk, d.tbase, d.tmask);
+            KF_HIP_TRY(hipGetLastError());
+        }
     }
     return KF_OK;
+}
+
+WinArgs win_args(const DevicePacked &d1, const DevicePacked &d2) {
+    WinArgs w;
+    w.tm1 = d1.tmask && d2.tmask ? d1.tmask : nullptr;
+    w.tm2 = w.tm1 ? d2.tmask : nullptr;
+    w.tb1 = d1.tbase;
+    w.tb2 = d2.tbase;
+    w.perm1 = d1.perm;
+    w.perm2 = d2.perm;
+    w.nblk1 = d1.nblk;
+    w.nblk2 = d2.nblk;
+    w.nwin = d1.rep_pad / KS;
+    return w;
 }
 
 // Output-block order: 8 x 8 super-blocks of blocks, super-block rows then columns,
@@ -463,12 +735,24 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     KF_REQUIRE(out, KF_ERR_ARG, "null output");
     std::vector<int> hQ, hN;
     if ((rc = read_metadata(Q, N, nm, ma, hQ, hN, stream, WS_AUX0))) return rc;
+    const std::vector<int> labels = distinct_labels(hQ.data(), hN.data(), nm, ma);
     LocalPlan p;
-    build_plan(hQ.data(), hN.data(), nm, ma, distinct_labels(hQ.data(), hN.data(), nm, ma), p);
-    if (precision == KF_PREC_BF16X3)
+    if (precision == KF_PREC_BF16X3) {
+        build_plan(hQ.data(), hN.data(), nm, ma, labels, p);
         return local_mixed_symm(MODE == SYMM_DENSE, X, nm, ma, rep, sigma, p, out, stream);
+    }
+    std::vector<int> perm, Qp, Np;
+    if (windows_masked(rep)) {
+        std::vector<unsigned long long> mm;
+        if ((rc = molecule_windows(X, N, nm, ma, rep, WS_AUX2, stream, mm))) return rc;
+        perm = order_molecules(mm, rank_windows({&mm}, padded_rep(rep) / KS));
+        permute_metadata(hQ, hN, ma, perm, Qp, Np);
+        build_plan(Qp.data(), Np.data(), nm, ma, labels, p);
+    } else {
+        build_plan(hQ.data(), hN.data(), nm, ma, labels, p);
+    }
     DevicePacked d;
-    if ((rc = upload_and_pack(p, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
+    if ((rc = upload_and_pack(p, perm, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
     const int nblk = (nm + MB - 1) / MB;
     const long ntile = (long)nblk * (nblk + 1) / 2;
     const double cexp = -1.0 / (2.0 * sigma * sigma);
@@ -482,7 +766,8 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3((unsigned)ntile), dim3(NT), 0, stream,
                        d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
-                       p.nlabels, nm, nm, d.rep_pad, cexp, order, static_cast<double *>(out));
+                       p.nlabels, nm, nm, d.rep_pad, cexp, order, win_args(d, d),
+                       static_cast<double *>(out));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
@@ -530,14 +815,32 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
         return KF_OK;
     }
     LocalPlan p1, p2;
-    build_plan(q1.data(), n1.data(), nm1, max_atoms1, labels, p1);
-    build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
-    if (precision == KF_PREC_BF16X3)
+    if (precision == KF_PREC_BF16X3) {
+        build_plan(q1.data(), n1.data(), nm1, max_atoms1, labels, p1);
+        build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
         return local_mixed_rect(X1, X2, nm1, nm2, max_atoms1, max_atoms2, rep, sigma, p1, p2,
                                 static_cast<float *>(K), stream);
+    }
+    std::vector<int> perm1, perm2;
+    if (windows_masked(rep)) {
+        std::vector<unsigned long long> m1, m2;
+        if ((rc = molecule_windows(X1, N1, nm1, max_atoms1, rep, WS_AUX2, stream, m1))) return rc;
+        if ((rc = molecule_windows(X2, N2, nm2, max_atoms2, rep, WS_AUX3, stream, m2))) return rc;
+        const std::vector<int> rank = rank_windows({&m1, &m2}, padded_rep(rep) / KS);
+        perm1 = order_molecules(m1, rank);
+        perm2 = order_molecules(m2, rank);
+        std::vector<int> qp, np;
+        permute_metadata(q1, n1, max_atoms1, perm1, qp, np);
+        build_plan(qp.data(), np.data(), nm1, max_atoms1, labels, p1);
+        permute_metadata(q2, n2, max_atoms2, perm2, qp, np);
+        build_plan(qp.data(), np.data(), nm2, max_atoms2, labels, p2);
+    } else {
+        build_plan(q1.data(), n1.data(), nm1, max_atoms1, labels, p1);
+        build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
+    }
     DevicePacked d1, d2;
-    if ((rc = upload_and_pack(p1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
-    if ((rc = upload_and_pack(p2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
+    if ((rc = upload_and_pack(p1, perm1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
+    if ((rc = upload_and_pack(p2, perm2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
     const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
     const double cexp = -1.0 / (2.0 * sigma * sigma);
     const int2 *order = nullptr;
@@ -546,7 +849,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3((unsigned)((long)nblk1 * nblk2)),
                        dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
                        d2.n2, d2.mol_row, d2.src, max_atoms2, (int)labels.size(), nm1, nm2,
-                       d1.rep_pad, cexp, order, static_cast<double *>(K));
+                       d1.rep_pad, cexp, order, win_args(d1, d2), static_cast<double *>(K));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }

@@ -43,6 +43,49 @@ def test_rect_random(lk, nm1, nm2):
     assert rel_err(K, oracle.kernel_rect(x1, x2, q1, q2, n1, n2, 0.9)) < 1e-12
 
 
+def windowed_set(nm, ma, rep, nlab, seed, ngroups=7):
+    """Random rows whose columns fall into groups (spans not aligned to the kernel's
+    16-column windows) that each molecule uses or leaves at zero, single atoms dropping
+    further groups, and a few all-zero molecules: the pattern of FCHL19 channels of
+    absent neighbour elements, at random."""
+    rng = np.random.default_rng(seed)
+    x, q, n = random_set(nm, ma, rep, nlab, seed)
+    cuts = np.sort(rng.choice(np.arange(1, rep), ngroups - 1, replace=False))
+    spans = np.split(np.arange(rep), cuts)
+    use = rng.random((nm, ngroups)) < 0.45
+    use[:, 0] = True
+    for a in range(nm):
+        for g, cols in enumerate(spans):
+            if not use[a, g]:
+                x[a][:, cols] = 0.0
+        for i in range(ma):
+            if rng.random() < 0.2:
+                x[a, i, spans[rng.integers(ngroups)]] = 0.0
+    x[rng.random(nm) < 0.05] = 0.0
+    return x, q, n
+
+
+@pytest.mark.parametrize("nm,ma,rep,nlab", [(300, 7, 720, 3), (200, 5, 1500, 2), (150, 4, 2100, 2),
+                                            (129, 6, 40, 2)])
+def test_symm_zero_windows(lk, nm, ma, rep, nlab):
+    """Molecules are reordered by the windows they use and zero windows are skipped per
+    tile pair (descriptors up to 128 windows of 16 columns; 2100 columns run unmasked)."""
+    x, q, n = windowed_set(nm, ma, rep, nlab, rep + nm)
+    sigma = 0.35 * np.sqrt(rep)
+    assert rel_err(lk.kernel_gaussian_symm(x, q, n, sigma), oracle.kernel_symm(x, q, n, sigma)) < 1e-12
+    assert rel_err(lk.kernel_gaussian_symm_rfp(x, q, n, sigma),
+                   oracle.kernel_symm_rfp(x, q, n, sigma)) < 1e-12
+
+
+@pytest.mark.parametrize("nm1,nm2,rep", [(250, 190, 720), (70, 130, 1100)])
+def test_rect_zero_windows(lk, nm1, nm2, rep):
+    x1, q1, n1 = windowed_set(nm1, 6, rep, 3, 11)
+    x2, q2, n2 = windowed_set(nm2, 5, rep, 3, 12)
+    sigma = 0.35 * np.sqrt(rep)
+    K = lk.kernel_gaussian(x1, x2, q1, q2, n1, n2, sigma)
+    assert rel_err(K, oracle.kernel_rect(x1, x2, q1, q2, n1, n2, sigma)) < 1e-12
+
+
 def test_config0_descriptors(lk, config0):
     _, _, _, C, Q, N = config0
     X = oracle.fchl19_batch(C, Q, N)
