@@ -75,6 +75,33 @@ def test_symm_forms(lk, case):
         assert rel_err(got, ref) < TOL, name
 
 
+def test_zero_columns(lk):
+    """Columns that are zero in X and in the Jacobian for every atom of both sets are
+    dropped before the kernels run; columns zero in X but not in the Jacobian stay (their
+    Jacobian rows still enter the force-force blocks)."""
+    X1, dX1, Q1, N1 = rand_set(4, 6, 50, 2, 31)
+    X2, dX2, Q2, N2 = rand_set(5, 5, 50, 2, 32)
+    x_only = [3, 4, 17, 30, 31, 32]     # zero in X, Jacobian kept
+    both = [0, 1, 2, 10, 11, 40, 49]    # zero in X and in the Jacobian
+    for X, dX in ((X1, dX1), (X2, dX2)):
+        X[..., x_only + both] = 0.0
+        dX[..., both, :] = 0.0
+    dX2[..., 5, :] = 0.0                # zero in one set's Jacobian only: kept
+    s = 1.3
+    assert rel_err(lk.kernel_gaussian_full(X1, dX1, X2, dX2, Q1, Q2, N1, N2, s),
+                   oracle.full(X1, dX1, X2, dX2, Q1, Q2, N1, N2, s)) < TOL
+    assert rel_err(lk.kernel_gaussian_jacobian(X1, X2, dX2, Q1, Q2, N1, N2, s),
+                   oracle.jacobian(X1, X2, dX2, Q1, Q2, N1, N2, s)) < TOL
+    got = lk.kernel_gaussian_full_symm_rfp(X1, dX1, Q1, N1, s)
+    assert rel_err(got, oracle.full_symm_rfp(X1, dX1, Q1, N1, s)) < TOL
+    got = lk.kernel_gaussian_hessian_symm(X2, dX2, Q2, N2, s)
+    assert rel_err(got, oracle.hessian_symm(X2, dX2, Q2, N2, s)) < TOL
+    # every column zero: the kernel of the constant descriptor
+    Z, dZ = np.zeros_like(X1), np.zeros_like(dX1)
+    assert rel_err(lk.kernel_gaussian_full_symm(Z, dZ, Q1, N1, s),
+                   oracle.full_symm(Z, dZ, Q1, N1, s)) < TOL
+
+
 def test_full_symm_conformers(lk):
     """Real FCHL19 descriptors + Jacobians of 10 C9H8O4 conformers: the RFP E+F
     kernel, its dense form, and the block identities of the full kernel."""
