@@ -30,7 +30,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["kf_runtime.hip", "fchl19.hip", "local_scalar.hip", "local_mixed.hip", "rfp_chol.hip",
                "rfp_convert.hip", "rfp_solve.hip", "local_deriv.hip",
-               "local_matvec.hip", "ef_pcg.hip", "mlp.hip"]
+               "local_matvec.hip", "ef_pcg.hip", "mlp.hip", "feature_compact.hip"]
 PY_MODULES = {"fchl19_repr": "py_fchl19_repr.cpp", "local_kernels": "py_local_kernels.cpp",
               "kernelmath": "py_kernelmath.cpp"}
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <vector>
 
+#include "feature_compact.hpp"
 #include "kf_common.hpp"
 #include "local_plan.hpp"
 
@@ -405,11 +406,17 @@ extern "C" int kf_ef_pcg_matfree(const double *X, const double *dX, const int *Q
     Vec vec;
     int rc = vec.init(st);
     if (rc) return rc;
+    // every matvec runs on the descriptor columns the set uses (exact: the others are zero)
+    CompactSets cs;
+    cs.x1 = cs.x2 = X; cs.dx1 = cs.dx2 = dX; cs.n1 = cs.n2 = N;
+    cs.nm1 = cs.nm2 = nm; cs.ma1 = cs.ma2 = max_atoms;
+    int nf = rep;
+    if ((rc = compact_columns(cs, rep, WS_PACK, WS_PACK, st, &nf))) return rc;
     MatFreeOp A;
-    A.X = X; A.dX = dX; A.Q = Q; A.N = N; A.nm = nm; A.ma = max_atoms; A.rep = rep;
+    A.X = cs.x1; A.dX = cs.dx1; A.Q = Q; A.N = N; A.nm = nm; A.ma = max_atoms; A.rep = nf;
     A.n = (int)n; A.sigma = sigma; A.lambda = lambda; A.vec = &vec; A.st = st;
     A.ad = static_cast<double *>(
-        workspace(WS_PACK2, sizeof(double) * (size_t)nm * max_atoms * rep, st));
+        workspace(WS_PACK2, sizeof(double) * (size_t)nm * max_atoms * nf, st));
     KF_REQUIRE(A.ad, KF_ERR_NOMEM, "workspace allocation failed");
     BlockPrec P;
     P.inv = inv; P.N = N; P.offs = d_offs; P.nb = nm; P.bs = bs; P.ma = max_atoms; P.nm = nm;

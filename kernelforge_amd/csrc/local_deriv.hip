@@ -28,6 +28,7 @@
 #include <cmath>
 #include <vector>
 
+#include "feature_compact.hpp"
 #include "kf_common.hpp"
 #include "local_plan.hpp"
 
@@ -948,129 +949,18 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
     return KF_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Feature compaction.  A descriptor column that is zero for every atom of both sets, in X
-// and in the Jacobian, adds nothing to any entry of these kernels (the FCHL19 channels of
-// neighbour elements a data set does not contain: N and S for C9H8O4 conformers), so the
-// sets are gathered once onto their non-zero columns and the kernels run on those.
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void fill_int_kernel(int *p, int n, int v) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// flag[f] = 1 when column f is non-zero in X or dX for some atom of the set.  One
-// work-group per molecule; dynamic LDS of rep ints.
-__global__ __launch_bounds__(256) void active_cols_kernel(const double *__restrict__ X,
-                                                          const double *__restrict__ dX,
-                                                          const int *__restrict__ N, int ma,
-                                                          int rep, int *__restrict__ flag) {
-    extern __shared__ int lf[];
-    const int a = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int f = tid; f < rep; f += 256) lf[f] = 0;
-    __syncthreads();
-    const int na = clampi(N[a], 0, ma), nr = na * rep;
-    const double *x = X + (size_t)a * ma * rep;
-    for (int e = tid; e < nr; e += 256)
-        if (x[e] != 0.0) lf[e % rep] = 1;
-    if (dX) {
-        const int L = 3 * ma;  // a column of one atom's Jacobian is a run of L coordinates
-        const double *j = dX + (size_t)a * ma * rep * L;
-        for (int r = wave; r < nr; r += 4) {
-            bool nz = false;
-            for (int c = lane; c < L; c += 64) nz |= j[(size_t)r * L + c] != 0.0;
-            if (__ballot(nz) && lane == 0) lf[r % rep] = 1;
-        }
-    }
-    __syncthreads();
-    for (int f = tid; f < rep; f += 256)
-        if (lf[f]) flag[f] = 1;
-}
-
-// Y[r][g][c] = X[r][cols[g]][c]: row r of rep runs of L doubles -> nf runs.  One
-// work-group per row.
-__global__ __launch_bounds__(256) void gather_cols_kernel(const double *__restrict__ X, int rep,
-                                                          int L, const int *__restrict__ cols,
-                                                          int nf, double *__restrict__ Y) {
-    const size_t r = blockIdx.x;
-    const double *src = X + r * rep * L;
-    double *dst = Y + r * nf * L;
-    const int n = nf * L;
-    for (int e = threadIdx.x; e < n; e += 256) {
-        const int g = e / L, c = e - g * L;
-        dst[e] = src[(size_t)cols[g] * L + c];
-    }
-}
-
-// Gather X (and dX, if given) of one set onto the columns cols (device) into ws.
-int gather_set(const double *X, const double *dX, int nm, int ma, int rep, const int *cols,
-               int nf, int ws, hipStream_t stream, const double **Xc, const double **dXc) {
-    const size_t rows = (size_t)nm * ma, L = 3 * (size_t)ma;
-    const size_t nx = rows * nf, nj = dX ? rows * nf * L : 0;
-    double *buf = static_cast<double *>(workspace(ws, sizeof(double) * (nx + nj + 1), stream));
-    KF_REQUIRE(buf, KF_ERR_NOMEM, "workspace allocation failed (%zu B)", sizeof(double) * (nx + nj));
-    *Xc = buf;
-    *dXc = dX ? buf + nx : nullptr;
-    if (rows == 0) return KF_OK;
-    hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)rows), dim3(256), 0, stream, X, rep, 1,
-                       cols, nf, buf);
-    KF_HIP_TRY(hipGetLastError());
-    if (dX) {
-        hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)rows), dim3(256), 0, stream, dX, rep,
-                           (int)L, cols, nf, buf + nx);
-        KF_HIP_TRY(hipGetLastError());
-    }
-    return KF_OK;
-}
-
-// Point A at copies of the sets restricted to the columns that are non-zero in either set
-// (A unchanged when every column is used).
+// Drop the descriptor columns that are zero for every atom of both sets (in X and in the
+// Jacobians the mode reads): A then points at compacted copies.
 int compact_features(DerivArgs &A, const int *N1, const int *N2, int nm1, int nm2,
                      hipStream_t stream) {
-    const int rep = A.rep;
-    const bool same = A.x1 == A.x2 && A.dx1 == A.dx2 && N1 == N2 && nm1 == nm2 && A.ma1 == A.ma2;
-    int *flag = static_cast<int *>(workspace(WS_AUX1, sizeof(int) * (2 * (size_t)rep + 1), stream));
-    int *hflag = static_cast<int *>(host_staging(WS_AUX1, sizeof(int) * rep));
-    KF_REQUIRE(flag && hflag, KF_ERR_NOMEM, "workspace allocation failed");
-    hipLaunchKernelGGL(fill_int_kernel, dim3((rep + 255) / 256), dim3(256), 0, stream, flag, rep, 0);
-    KF_HIP_TRY(hipGetLastError());
-    const size_t lds = sizeof(int) * rep;
-    KF_REQUIRE(lds <= 65536, KF_ERR_ARG, "descriptor size %d too large", rep);
-    hipLaunchKernelGGL(active_cols_kernel, dim3(nm1), dim3(256), lds, stream, A.x1, A.dx1, N1,
-                       A.ma1, rep, flag);
-    KF_HIP_TRY(hipGetLastError());
-    if (!same) {
-        hipLaunchKernelGGL(active_cols_kernel, dim3(nm2), dim3(256), lds, stream, A.x2, A.dx2, N2,
-                           A.ma2, rep, flag);
-        KF_HIP_TRY(hipGetLastError());
-    }
-    KF_HIP_TRY(hipMemcpyAsync(hflag, flag, sizeof(int) * rep, hipMemcpyDeviceToHost, stream));
-    KF_HIP_TRY(hipStreamSynchronize(stream));
-    std::vector<int> cols;
-    for (int f = 0; f < rep; ++f)
-        if (hflag[f]) cols.push_back(f);
-    if ((int)cols.size() == rep) return KF_OK;
-    if (cols.empty()) cols.push_back(0);  // all-zero sets: one (zero) column keeps rep >= 1
-    const int nf = (int)cols.size();
-    std::copy(cols.begin(), cols.end(), hflag);
-    int *dcols = flag + rep;
-    KF_HIP_TRY(hipMemcpyAsync(dcols, hflag, sizeof(int) * nf, hipMemcpyHostToDevice, stream));
-    int rc;
-    const double *x1 = nullptr, *dx1 = nullptr, *x2 = nullptr, *dx2 = nullptr;
-    if ((rc = gather_set(A.x1, A.dx1, nm1, A.ma1, rep, dcols, nf, WS_PACK, stream, &x1, &dx1)))
-        return rc;
-    if (same) {
-        x2 = x1;
-        dx2 = dx1;
-    } else if ((rc = gather_set(A.x2, A.dx2, nm2, A.ma2, rep, dcols, nf, WS_PACK2, stream, &x2,
-                                &dx2))) {
-        return rc;
-    }
-    A.x1 = x1; A.dx1 = dx1; A.x2 = x2; A.dx2 = dx2;
+    CompactSets c;
+    c.x1 = A.x1; c.dx1 = A.dx1; c.n1 = N1; c.nm1 = nm1; c.ma1 = A.ma1;
+    c.x2 = A.x2; c.dx2 = A.dx2; c.n2 = N2; c.nm2 = nm2; c.ma2 = A.ma2;
+    int nf = A.rep;
+    const int rc = compact_columns(c, A.rep, WS_PACK, WS_PACK2, stream, &nf);
+    if (rc) return rc;
+    A.x1 = c.x1; A.dx1 = c.dx1; A.x2 = c.x2; A.dx2 = c.dx2;
     A.rep = nf;
-    // the staging copy above must land before hflag is reused by a later call
-    KF_HIP_TRY(hipStreamSynchronize(stream));
     return KF_OK;
 }
 
