@@ -175,7 +175,11 @@ template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
     __shared__ T S[NB * TP];
     __shared__ T col[NB];
+    __shared__ T dinv[NB];
     if (*info) return;
+    // this wave is the whole critical path of the factorization: let it issue ahead of the
+    // trailing-update waves that share its CU
+    __builtin_amdgcn_s_setprio(3);
     const int i = threadIdx.x;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
@@ -197,8 +201,14 @@ __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, in
         const T d = sqrt(piv), rd = T(1) / d;
         r[j] = i == j ? d : r[j] * rd;
         col[i] = r[j];  // read back below by the same wave: LDS keeps its order
+        if (i == 0) dinv[j] = rd;
+        // all of column j first, then the updates: one wait for the whole batch
+        T cv[NB];
 #pragma unroll
-        for (int k = j + 1; k < NB; ++k) r[k] -= r[j] * col[k];
+        for (int k = j + 1; k < NB; ++k) cv[k] = col[k];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = j + 1; k < NB; ++k) r[k] -= r[j] * cv[k];
     }
     if (fail) {
         if (i == 0 && fail <= w) atomicCAS(info, 0, s0 + fail);
@@ -208,17 +218,20 @@ __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, in
     for (int c = 0; c < NB; ++c) S[i * TP + c] = c <= i ? r[c] : T(0);
     __syncthreads();
     store_tile<T, true>(a, t, w, w, S);
-    // column c = i of L^-1: x[q] = (delta_qc - sum_{k<q} L[q][k] x[k]) / L[q][q]
+    // column c = i of L^-1: x[q] = (delta_qc - sum_{k<q} L[q][k] x[k]) / L[q][q], with the
+    // reciprocals of the diagonal kept from the factorization
     T x[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-        T s0a = q == i ? T(1) : T(0), s1a = T(0);
+        T lq[NB];
 #pragma unroll
-        for (int k = 0; k < q; k += 2) {
-            s0a -= S[q * TP + k] * x[k];
-            if (k + 1 < q) s1a -= S[q * TP + k + 1] * x[k + 1];
-        }
-        x[q] = (s0a + s1a) / S[q * TP + q];
+        for (int k = 0; k < q; ++k) lq[k] = S[q * TP + k];
+        const T dq = dinv[q];
+        __builtin_amdgcn_sched_barrier(0);
+        T acc[4] = {q == i ? T(1) : T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int k = 0; k < q; ++k) acc[k & 3] -= lq[k] * x[k];
+        x[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * dq;
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q) inv[q * NB + i] = (q < w && i < w) ? x[q] : T(0);
@@ -317,24 +330,24 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
             const int idx = threadIdx.x + 256 * e;
             int rr, cc;
             if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            const T vi = a[ti.base + (size_t)min(rr, rni - 1) * ti.rs + (size_t)min(cc, w - 1) * ti.cs];
-            pi[e] = (rr < rni && cc < w) ? vi : T(0);
+            // raw (clamped) values: the padding is zeroed at the LDS store, so the loads
+            // are not waited for until then
+            pi[e] = a[ti.base + (size_t)min(rr, rni - 1) * ti.rs + (size_t)min(cc, w - 1) * ti.cs];
             if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            const T vk = a[tk.base + (size_t)min(rr, rnk - 1) * tk.rs + (size_t)min(cc, w - 1) * tk.cs];
-            pk[e] = (rr < rnk && cc < w) ? vk : T(0);
+            pk[e] = a[tk.base + (size_t)min(rr, rnk - 1) * tk.rs + (size_t)min(cc, w - 1) * tk.cs];
         }
     };
     auto sstore = [&](int J) {
-        const int c0 = P.start(J);
+        const int c0 = P.start(J), w = P.width(J);
         const Tile ti = tile_at(P, ri, c0), tk = tile_at(P, rk, c0);
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int idx = threadIdx.x + 256 * e;
             int rr, cc;
             if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            Ai[rr * TP + cc] = pi[e];
+            Ai[rr * TP + cc] = (rr < rni && cc < w) ? pi[e] : T(0);
             if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            Ak[rr * TP + cc] = pk[e];
+            Ak[rr * TP + cc] = (rr < rnk && cc < w) ? pk[e] : T(0);
         }
     };
     gload(Jlo);
@@ -385,7 +398,11 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
 // block, columns past a panel's width read a valid column and are zeroed), so the
 // wait counts are exact.  In the column-major region the waves compute C^T so that
 // consecutive lanes touch contiguous rows.
-constexpr int SK = 16;       // chunk depth
+// Chunk depth of the trailing update.  8 keeps two work-groups at 36 KB of LDS each, so a
+// CU running two of them still has room for a panel-factorization kernel (the critical
+// path of the look-ahead) instead of holding it back until one of them retires.
+#define KF_SYRK_SK 8
+constexpr int SK = KF_SYRK_SK;
 constexpr int SP = SK + 1;   // LDS pitch
 
 struct PairList {  // pairs of blocks from Kbeg: lower region first, then upper
@@ -450,64 +467,94 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
         rmax[b] = P.width(bb) - 1;
         src[b] = cm ? a + (size_t)(P.k + 1 + r0) : a + (size_t)(r0 - P.k) * s;
     }
-    const int rc = tid & 63, cc_c = tid >> 6, rr_r = tid >> 4, cc_r = tid & 15;
-    T st[4][4];
-#define KF_SYRK_LOAD(J, kc)                                                                       \
+    // staged elements per thread and block: column-major blocks take row tid % 64 and
+    // columns tid / 64 + 4e, row-major blocks column tid % SK and rows tid / SK + RS e
+    constexpr int SE = NB * SK / 256, RS = 256 / SK;
+    const int rc = tid & 63, cc_c = tid >> 6, rr_r = tid / SK, cc_r = tid % SK;
+    T st[4][SE];
+    // one staged block b of chunk kc of panel column block J: the loads are unconditional
+    // (clamped addresses); columns past the block's width are zeroed when the values are
+    // stored to LDS, so nothing waits on a load until then
+#define KF_SYRK_LOAD(b, J, kc)                                                                    \
     do {                                                                                          \
         const int c0_ = P.start(J), wl_ = P.width(J) - 1;                                         \
-        _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                           \
-            const bool cm_ = b < 2 ? CMI : CMK;                                                   \
-            _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                       \
-                const int r_ = cm_ ? rc : rr_r + 16 * e;                                          \
-                const int c_ = (kc) * SK + (cm_ ? cc_c + 4 * e : cc_r);                           \
-                const int rcl = min(r_, rmax[b]), ccl = c0_ + min(c_, wl_);                       \
-                const T v_ = cm_ ? src[b][rcl + (size_t)ccl * s] : src[b][(size_t)rcl * s + ccl]; \
-                st[b][e] = c_ <= wl_ ? v_ : T(0);                                                 \
-            }                                                                                     \
+        constexpr bool cm_ = (b) < 2 ? CMI : CMK;                                                 \
+        _Pragma("unroll") for (int e = 0; e < SE; ++e) {                                          \
+            const int r_ = cm_ ? rc : rr_r + RS * e;                                              \
+            const int c_ = (kc) * SK + (cm_ ? cc_c + 4 * e : cc_r);                               \
+            const int rcl = min(r_, rmax[b]), ccl = c0_ + min(c_, wl_);                           \
+            st[b][e] = cm_ ? src[b][rcl + (size_t)ccl * s] : src[b][(size_t)rcl * s + ccl];       \
         }                                                                                         \
     } while (0)
-#define KF_SYRK_STORE(buf)                                                                        \
+#define KF_SYRK_STORE(buf, b, J, kc)                                                              \
     do {                                                                                          \
-        _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                           \
-            const bool cm_ = b < 2 ? CMI : CMK;                                                   \
-            _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                       \
-                const int r_ = cm_ ? rc : rr_r + 16 * e;                                          \
-                const int c_ = cm_ ? cc_c + 4 * e : cc_r;                                         \
-                (buf)[(b * NB + r_) * SP + c_] = st[b][e];                                        \
-            }                                                                                     \
+        const int wl_ = P.width(J) - 1;                                                           \
+        constexpr bool cm_ = (b) < 2 ? CMI : CMK;                                                 \
+        _Pragma("unroll") for (int e = 0; e < SE; ++e) {                                          \
+            const int r_ = cm_ ? rc : rr_r + RS * e;                                              \
+            const int c_ = cm_ ? cc_c + 4 * e : cc_r;                                             \
+            (buf)[((b) * NB + r_) * SP + c_] = (kc) * SK + c_ <= wl_ ? st[b][e] : T(0);          \
         }                                                                                         \
     } while (0)
+#define KF_SYRK_ALL(X) X(0); X(1); X(2); X(3)
     const int nch = (Jhi - Jlo) * (NB / SK);
-    KF_SYRK_LOAD(Jlo, 0);
-    KF_SYRK_STORE(Lb);
-    if (nch > 1) KF_SYRK_LOAD(Jlo, 1);
+#define KF_L0(b) KF_SYRK_LOAD(b, Jlo, 0)
+#define KF_L1(b) KF_SYRK_LOAD(b, Jlo, 1)
+#define KF_S0(b) KF_SYRK_STORE(Lb, b, Jlo, 0)
+    KF_SYRK_ALL(KF_L0);
+    KF_SYRK_ALL(KF_S0);
+    if (nch > 1) { KF_SYRK_ALL(KF_L1); }
+#undef KF_L0
+#undef KF_L1
+#undef KF_S0
     __syncthreads();
     // rows of the computed tile (X) and its columns (Y) inside a buffer
     const int xo = trans ? (2 + wj) * NB : wi * NB, yo = trans ? wi * NB : (2 + wj) * NB;
     for (int ch = 0; ch < nch; ++ch) {
         const T *cur = Lb + (ch & 1) * (4 * NB * SP);
-        if (ch + 1 < nch) {
-            KF_SYRK_STORE(Lb + ((ch + 1) & 1) * (4 * NB * SP));
-            const int c2 = min(ch + 2, nch - 1);
-            KF_SYRK_LOAD(Jlo + c2 / (NB / SK), c2 % (NB / SK));
-        }
-        if (active) {
-#pragma unroll
-            for (int k0 = 0; k0 < SK; k0 += 4) {
-                const int kk = k0 + (lane >> 4);
-                T av[4], bv[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) av[i] = cur[(xo + i * 16 + (lane & 15)) * SP + kk];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) bv[j] = cur[(yo + j * 16 + (lane & 15)) * SP + kk];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(-av[i], bv[j], acc[i][j]);
-            }
-        }
+        T *nxt = Lb + ((ch + 1) & 1) * (4 * NB * SP);
+        // the last chunk re-stages itself into the idle buffer (never read): every load and
+        // store is unconditional, so the wait counts stay exact across iterations
+        const int c1 = min(ch + 1, nch - 1), c2 = min(ch + 2, nch - 1);
+        const int J1 = Jlo + c1 / (NB / SK), kc1 = c1 % (NB / SK);
+        const int J2 = Jlo + c2 / (NB / SK), kc2 = c2 % (NB / SK);
+        // k-step ks of this chunk, then staged blocks of the next chunk moved to LDS and the
+        // same blocks of the chunk after it fetched: the staging spreads over the MFMAs
+#define KF_SYRK_MMA(ks)                                                                           \
+        do {                                                                                      \
+            if (active) {                                                                         \
+                const int kk = 4 * (ks) + (lane >> 4);                                            \
+                T av[4], bv[4];                                                                   \
+                _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
+                    av[i] = cur[(xo + i * 16 + (lane & 15)) * SP + kk];                           \
+                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                     \
+                    bv[j] = cur[(yo + j * 16 + (lane & 15)) * SP + kk];                           \
+                _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
+                    _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
+                        acc[i][j] = mfma4(-av[i], bv[j], acc[i][j]);                              \
+            }                                                                                     \
+        } while (0)
+#define KF_SYRK_STAGE(b)                                                                          \
+        do {                                                                                      \
+            KF_SYRK_STORE(nxt, b, J1, kc1);                                                       \
+            KF_SYRK_LOAD(b, J2, kc2);                                                             \
+        } while (0)
+#if KF_SYRK_SK == 16
+        KF_SYRK_MMA(0); KF_SYRK_STAGE(0);
+        KF_SYRK_MMA(1); KF_SYRK_STAGE(1);
+        KF_SYRK_MMA(2); KF_SYRK_STAGE(2);
+        KF_SYRK_MMA(3); KF_SYRK_STAGE(3);
+#elif KF_SYRK_SK == 8
+        KF_SYRK_MMA(0); KF_SYRK_STAGE(0); KF_SYRK_STAGE(1);
+        KF_SYRK_MMA(1); KF_SYRK_STAGE(2); KF_SYRK_STAGE(3);
+#else
+#error "trailing-update chunk depth must be 8 or 16"
+#endif
+#undef KF_SYRK_MMA
+#undef KF_SYRK_STAGE
         __syncthreads();
     }
+#undef KF_SYRK_ALL
 #undef KF_SYRK_LOAD
 #undef KF_SYRK_STORE
     if (!active) return;
