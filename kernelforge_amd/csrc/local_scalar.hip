@@ -358,11 +358,12 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     __shared__ int mrA[MB + 1], mrB[MB + 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // work-group -> output block through the host-built order (8 x 8 super-blocks of
-    // blocks); the hardware deals work-groups round-robin over the 8 XCDs, so logical
-    // slot (g % 8) * (G / 8) + g / 8 gives each XCD a contiguous run of the order.
+    // work-group -> output block through the host-built order: the hardware deals
+    // work-groups round-robin over the 8 XCDs, so logical slot (g % 8) * (G / 8) + g / 8
+    // gives each XCD a contiguous run of the order (its list of column-block groups).
     const int gi = blockIdx.x, G = gridDim.x;
     const int2 ab = order[(G % 8 == 0) ? (gi % 8) * (G / 8) + gi / 8 : gi];
+    if (ab.x < 0) return;  // padding of a shorter XCD list
     const int A = ab.x, B = ab.y;
     const int a0 = A * MB, na = min(MB, nm1 - a0);
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
@@ -695,17 +696,36 @@ WinArgs win_args(const DevicePacked &d1, const DevicePacked &d2) {
     return w;
 }
 
-// Output-block order: 8 x 8 super-blocks of blocks, super-block rows then columns,
-// blocks inside a super-block row-major (upper triangle only when symmetric).
-int upload_block_order(int nb1, int nb2, bool symm, hipStream_t stream, const int2 **dev) {
-    constexpr int S = 8;
+// Output-block order.  Work-group g runs on XCD g % 8 (the hardware deals work-groups
+// round-robin over the 8 XCDs), and the kernel maps it to logical slot
+// (g % 8) * (G / 8) + g / 8, so each XCD walks its own contiguous list.  The list of an XCD
+// is a sequence of column-block groups -- every row block A paired with one column block B
+// -- so the work-groups resident on an XCD at one time share their column operands (the
+// larger operand stream, read by every wave straight from memory) through its L2.  Column
+// blocks are dealt to the XCDs largest group first, each to the list with the fewest
+// pairs; lists are padded to one length with (-1, -1) entries, which exit at once.
+int upload_block_order(int nb1, int nb2, bool symm, hipStream_t stream, const int2 **dev,
+                       unsigned *grid) {
+    constexpr int X = 8;
+    std::vector<std::vector<int2>> lists(X);
+    std::vector<int> cols(nb2);
+    for (int b = 0; b < nb2; ++b) cols[b] = b;
+    if (symm) std::reverse(cols.begin(), cols.end());  // largest groups first
+    for (int b : cols) {
+        size_t x = 0;
+        for (size_t y = 1; y < lists.size(); ++y)
+            if (lists[y].size() < lists[x].size()) x = y;
+        const int na = symm ? b + 1 : nb1;
+        for (int a = 0; a < na; ++a) lists[x].push_back(make_int2(a, b));
+    }
+    size_t len = 0;
+    for (const auto &l : lists) len = std::max(len, l.size());
     std::vector<int2> order;
-    order.reserve(symm ? (size_t)nb1 * (nb1 + 1) / 2 : (size_t)nb1 * nb2);
-    for (int P = 0; P < nb1; P += S)
-        for (int Q = symm ? P : 0; Q < nb2; Q += S)
-            for (int a = P; a < std::min(P + S, nb1); ++a)
-                for (int b = Q; b < std::min(Q + S, nb2); ++b)
-                    if (!symm || a <= b) order.push_back(make_int2(a, b));
+    order.reserve(X * len);
+    for (auto &l : lists) {
+        l.resize(len, make_int2(-1, -1));
+        order.insert(order.end(), l.begin(), l.end());
+    }
     const size_t bytes = order.size() * sizeof(int2);
     int2 *d = static_cast<int2 *>(workspace(WS_AUX1, bytes, stream));
     int2 *h = static_cast<int2 *>(host_staging(WS_AUX1, bytes));
@@ -713,6 +733,7 @@ int upload_block_order(int nb1, int nb2, bool symm, hipStream_t stream, const in
     std::copy(order.begin(), order.end(), h);
     KF_HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
     *dev = d;
+    *grid = (unsigned)order.size();
     return KF_OK;
 }
 
@@ -754,7 +775,6 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     DevicePacked d;
     if ((rc = upload_and_pack(p, perm, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
     const int nblk = (nm + MB - 1) / MB;
-    const long ntile = (long)nblk * (nblk + 1) / 2;
     const double cexp = -1.0 / (2.0 * sigma * sigma);
     if (p.nlabels == 0) {  // no atoms at all: zero output
         const size_t n = MODE == SYMM_RFP ? (size_t)nm * (nm + 1) / 2 : (size_t)nm * nm;
@@ -762,9 +782,10 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
         return KF_OK;
     }
     const int2 *order = nullptr;
-    if ((rc = upload_block_order(nblk, nblk, true, stream, &order))) return rc;
+    unsigned grid = 0;
+    if ((rc = upload_block_order(nblk, nblk, true, stream, &order, &grid))) return rc;
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
-    hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3((unsigned)ntile), dim3(NT), 0, stream,
+    hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3(grid), dim3(NT), 0, stream,
                        d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
                        p.nlabels, nm, nm, d.rep_pad, cexp, order, win_args(d, d),
                        static_cast<double *>(out));
@@ -844,9 +865,10 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
     const double cexp = -1.0 / (2.0 * sigma * sigma);
     const int2 *order = nullptr;
-    if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order))) return rc;
+    unsigned grid = 0;
+    if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order, &grid))) return rc;
     ScopedTimer timer(TM_LOCAL_RECT, stream);
-    hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3((unsigned)((long)nblk1 * nblk2)),
+    hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3(grid),
                        dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
                        d2.n2, d2.mol_row, d2.src, max_atoms2, (int)labels.size(), nm1, nm2,
                        d1.rep_pad, cexp, order, win_args(d1, d2), static_cast<double *>(K));
