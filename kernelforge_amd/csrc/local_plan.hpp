@@ -34,7 +34,15 @@ inline void build_plan(const int *Q, const int *N, int nm, int ma, const std::ve
     p.mol_row.assign((size_t)L * (nm + 1), 0);
     // per (label, molecule) counts
     std::vector<int> cnt((size_t)L * nm, 0);
+    // label -> index: a direct table for the small non-negative labels atomic numbers are,
+    // a binary search for anything else
+    constexpr int kDirect = 256;
+    int direct[kDirect];
+    std::fill(direct, direct + kDirect, -1);
+    for (int l = 0; l < L; ++l)
+        if (labels[l] >= 0 && labels[l] < kDirect) direct[labels[l]] = l;
     auto lab_idx = [&](int q) {
+        if (q >= 0 && q < kDirect) return direct[q];
         auto it = std::lower_bound(labels.begin(), labels.end(), q);
         return (it != labels.end() && *it == q) ? int(it - labels.begin()) : -1;
     };
@@ -69,14 +77,31 @@ inline void build_plan(const int *Q, const int *N, int nm, int ma, const std::ve
     }
 }
 
+// Sorted distinct labels of the atoms (i < N[a]).  Labels in [0, 256) (atomic numbers)
+// are collected in a presence table; any others are sorted and merged in.
 inline std::vector<int> distinct_labels(const int *Q, const int *N, int nm, int ma) {
-    std::vector<int> out;
+    constexpr int kDirect = 256;
+    bool seen[kDirect] = {};
+    std::vector<int> other;
     for (int a = 0; a < nm; ++a) {
         const int na = clamp_atoms(N[a], ma);
-        for (int i = 0; i < na; ++i) out.push_back(Q[(size_t)a * ma + i]);
+        const int *q = Q + (size_t)a * ma;
+        for (int i = 0; i < na; ++i) {
+            if (q[i] >= 0 && q[i] < kDirect) seen[q[i]] = true;
+            else other.push_back(q[i]);
+        }
     }
-    std::sort(out.begin(), out.end());
-    out.erase(std::unique(out.begin(), out.end()), out.end());
+    std::vector<int> out;
+    for (int v = 0; v < kDirect; ++v)
+        if (seen[v]) out.push_back(v);
+    if (!other.empty()) {
+        std::sort(other.begin(), other.end());
+        other.erase(std::unique(other.begin(), other.end()), other.end());
+        std::vector<int> merged(out.size() + other.size());
+        merged.erase(std::merge(out.begin(), out.end(), other.begin(), other.end(), merged.begin()),
+                     merged.end());
+        out.swap(merged);
+    }
     return out;
 }
 
