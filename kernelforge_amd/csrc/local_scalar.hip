@@ -518,10 +518,20 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                         const int s1 = segl[u][ob], nrr = nrr_s;
                         for (int j = wave; j < nrr; j += 4) {
                             const int rs = rstart[j], re = rstart[j + 1];
-                            double sum = 0.0;
-                            for (int i = rs; i < re; ++i)
-                                for (int sg = s0; sg <= s1; ++sg) sum += P[i * PP + sg];
-                            accL[rowmol[rs] * AP + ob] += sum;
+                            // four independent partial sums over the rows, so the LDS reads
+                            // of a row molecule are in flight together
+                            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                            for (int sg = s0; sg <= s1; ++sg) {
+                                int i = rs;
+                                for (; i + 3 < re; i += 4) {
+                                    a0 += P[i * PP + sg];
+                                    a1 += P[(i + 1) * PP + sg];
+                                    a2 += P[(i + 2) * PP + sg];
+                                    a3 += P[(i + 3) * PP + sg];
+                                }
+                                for (; i < re; ++i) a0 += P[i * PP + sg];
+                            }
+                            accL[rowmol[rs] * AP + ob] += (a0 + a1) + (a2 + a3);
                         }
                     }
                 }
