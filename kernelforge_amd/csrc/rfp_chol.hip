@@ -736,7 +736,13 @@ __global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const
     }
 }
 
-constexpr int NBO = 4;  // 64-wide panels per outer (256-deep) trailing update
+// Outer panels of 64-wide blocks: 8 (a 512-deep trailing update) while more than
+// kWideFrom blocks remain, 4 (256-deep) after that.  A deeper update reads and writes the
+// trailing matrix half as often; near the end the panel chain is the critical path and
+// narrow panels keep it short.  Measured at orders 20,000 / 32,000 / 64,000: 75.4 /
+// 237.8 / 1680 ms against 74.9 / 248.0 / 1828 ms with 4 throughout; 16-wide panels at the
+// start are slower (246.8 ms at 32,000).
+constexpr int kWideFrom = 200;
 
 template <typename T>
 int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
@@ -751,7 +757,7 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     if (l2 != 0.0) {
         hipLaunchKernelGGL(add_diag<T>, dim3((n + 255) / 256), dim3(256), 0, stream, a, n, l2);
     }
-    // Look-ahead over outer panels of NBO blocks.  factor(p) = diagonal POTRF + inverse,
+    // Look-ahead over outer panels (widths from next()).  factor(p) = diagonal POTRF + inverse,
     // panel TRSM and the in-panel updates of panel p; it runs on the aux stream while
     // the caller's stream applies panel p-1 to everything right of panel p.  Per p:
     //   caller: wait factor(p); update panel p+1's columns from p (64-wide tiles); mark N
@@ -774,13 +780,17 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
             }
         }
     };
+    auto next = [&](int J) {  // end of the outer panel that starts at block J
+        const int w = P.nblk - J > kWideFrom ? 8 : 4;
+        return J + w < P.nblk ? J + w : P.nblk;
+    };
     KF_HIP_TRY(hipEventRecord(ev_start, stream));
     KF_HIP_TRY(hipStreamWaitEvent(s2, ev_start, 0));
-    factor(0, NBO < P.nblk ? NBO : P.nblk);
+    factor(0, next(0));
     KF_HIP_TRY(hipEventRecord(ev_f, s2));
-    for (int J0 = 0; J0 < P.nblk; J0 += NBO) {
-        const int J1 = J0 + NBO < P.nblk ? J0 + NBO : P.nblk;
-        const int J2 = J1 + NBO < P.nblk ? J1 + NBO : P.nblk;
+    for (int J0 = 0; J0 < P.nblk; J0 = next(J0)) {
+        const int J1 = next(J0);
+        const int J2 = J1 < P.nblk ? next(J1) : P.nblk;
         KF_HIP_TRY(hipStreamWaitEvent(stream, ev_f, 0));
         if (J1 < P.nblk) {
             const long nt = range_tiles(P, J1, J2);
@@ -791,7 +801,7 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
             factor(J1, J2);
             KF_HIP_TRY(hipEventRecord(ev_f, s2));
         }
-        if (J2 < P.nblk) {  // everything right of the next outer panel, NBO blocks deep
+        if (J2 < P.nblk) {  // everything right of the next outer panel, panel p deep
             const PairList L = make_pairs(P, J2);
             const long nt = (long)L.np * (L.np + 1) / 2;
             hipLaunchKernelGGL(syrk2_range<T>, dim3((unsigned)nt), dim3(256), 0, stream, a, P, L,
