@@ -59,6 +59,21 @@ def test_cho_solve_rfp_and_factor(km, n):
     assert rel_err(work, fac) < 1e-11
 
 
+def test_cho_solve_rfp_wide_panels(km):
+    """Order 16,000 (250 blocks of 64): the first outer panels are 512 columns wide (while
+    more than 200 blocks remain), the rest 256.  A symmetric, strictly diagonally dominant
+    matrix (cheap to build at this size); the solution is checked by its residual."""
+    n = 16000
+    rng = np.random.default_rng(16)
+    K = rng.random((n, n)) - 0.5
+    K = K + K.T
+    K[np.diag_indices(n)] += n
+    y = rng.standard_normal(n)
+    a = km.cho_solve_rfp(to_rfp(K), y, l2=0.0)
+    r = K @ a - y
+    assert np.linalg.norm(r) / np.linalg.norm(y) < 1e-12
+
+
 def test_not_spd_info_matches_lapack(km):
     n = 70
     K = spd(n, 3)
