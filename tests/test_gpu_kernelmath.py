@@ -130,3 +130,34 @@ def test_fp32_potrf_potrs_c_abi(n):
     capi.rfp_potrs(capi.KF_PREC_F32, a32.ptr, n, b.ptr, 1)
     capi.synchronize()
     assert rel_err(b.numpy(), np.linalg.solve(K, y)) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_potrs_repeated_on_same_buffers(prec):
+    """Repeated solves on the same factor and right-hand-side buffers with new right-hand
+    sides, with other solves and host synchronisations in between, each match numpy (the
+    inverse diagonal blocks and the intermediate vector live in a reused workspace)."""
+    from kernelforge_amd import capi
+
+    n, n2 = 700, 333
+    p = capi.KF_PREC_F64 if prec == "f64" else capi.KF_PREC_F32
+    dt = np.float64 if prec == "f64" else np.float32
+    tol = 1e-10 if prec == "f64" else 1e-4
+    K, K2 = spd(n, 7, shift=2.0), spd(n2, 8, shift=2.0)
+    A = capi.DeviceArray.from_numpy(to_rfp(K).astype(dt))
+    A2 = capi.DeviceArray.from_numpy(to_rfp(K2).astype(dt))
+    capi.rfp_potrf(p, A.ptr, n, 0.0)
+    capi.rfp_potrf(p, A2.ptr, n2, 0.0)
+    B = capi.DeviceArray((n,))
+    rng = np.random.default_rng(3)
+    for rep in range(4):
+        y = rng.standard_normal(n)
+        B.copy_from(capi.DeviceArray.from_numpy(y))
+        capi.rfp_potrs(p, A.ptr, n, B.ptr, 1)
+        capi.synchronize()
+        assert rel_err(B.numpy(), np.linalg.solve(K, y)) < tol, rep
+        y2 = rng.standard_normal(n2)  # another solve in between, on fresh buffers
+        B2 = capi.DeviceArray.from_numpy(y2)
+        capi.rfp_potrs(p, A2.ptr, n2, B2.ptr, 1)
+        capi.synchronize()
+        assert rel_err(B2.numpy(), np.linalg.solve(K2, y2)) < tol, rep
