@@ -90,6 +90,8 @@ int gather_set(const double *X, const double *dX, int nm, int ma, int rep, const
 int compact_columns(CompactSets &c, int rep, int ws1, int ws2, hipStream_t stream, int *nf_out) {
     *nf_out = rep;
     if (rep < 1 || (c.nm1 == 0 && c.nm2 == 0)) return KF_OK;
+    const size_t lds = sizeof(int) * rep;
+    if (lds > 65536) return KF_OK;  // wider than one work-group's flags: run on every column
     const bool same = c.x1 == c.x2 && c.dx1 == c.dx2 && c.n1 == c.n2 && c.nm1 == c.nm2 &&
                       c.ma1 == c.ma2;
     int *flag = static_cast<int *>(workspace(WS_AUX1, sizeof(int) * (2 * (size_t)rep + 1), stream));
@@ -97,8 +99,6 @@ int compact_columns(CompactSets &c, int rep, int ws1, int ws2, hipStream_t strea
     KF_REQUIRE(flag && hflag, KF_ERR_NOMEM, "workspace allocation failed");
     hipLaunchKernelGGL(fill_int_kernel, dim3((rep + 255) / 256), dim3(256), 0, stream, flag, rep, 0);
     KF_HIP_TRY(hipGetLastError());
-    const size_t lds = sizeof(int) * rep;
-    KF_REQUIRE(lds <= 65536, KF_ERR_ARG, "descriptor size %d too large", rep);
     if (c.nm1 > 0) {
         hipLaunchKernelGGL(active_cols_kernel, dim3(c.nm1), dim3(256), lds, stream, c.x1, c.dx1,
                            c.n1, c.ma1, rep, flag);
