@@ -213,17 +213,19 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
     constexpr int NSUB = KA / KS;
     constexpr int NB = NTW > 0 ? NTW : 1;
     static_assert(KS == 16 && NSUB == 2, "two staged row parts and two ring slots per chunk");
-    double2 ra0, ra1;         // one staged row part: 4 columns of one sub-chunk
+    // the staged row chunk, 4 columns of each of its two sub-chunks (named registers: an
+    // array here is left in scratch memory by the compiler)
+    double2 ra0_0, ra0_1, ra1_0, ra1_1;
     double2 bq[NSUB][NB][2];  // column ring: sub-chunk gs in slot gs % 2
-#define KF_LOAD_A(k)                                       \
-    do {                                                   \
-        ra0 = *(const double2 *)(g.pa + (k));              \
-        ra1 = *(const double2 *)(g.pa + (k) + 2);          \
+#define KF_LOAD_A(p, k)                                          \
+    do {                                                         \
+        ra##p##_0 = *(const double2 *)(g.pa + (k));              \
+        ra##p##_1 = *(const double2 *)(g.pa + (k) + 2);          \
     } while (0)
-#define KF_STORE_A(buf, s)                                 \
-    do {                                                   \
-        *(double2 *)((buf) + g.lofs + (s) * KS) = ra0;     \
-        *(double2 *)((buf) + g.lofs + (s) * KS + 2) = ra1; \
+#define KF_STORE_A(buf, p)                                       \
+    do {                                                         \
+        *(double2 *)((buf) + g.lofs + (p) * KS) = ra##p##_0;     \
+        *(double2 *)((buf) + g.lofs + (p) * KS + 2) = ra##p##_1; \
     } while (0)
 #define KF_LOAD_B(slot, k)                                               \
     do {                                                                 \
@@ -239,11 +241,10 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
         __syncthreads();
         return;
     }
-#pragma unroll
-    for (int s = 0; s < NSUB; ++s) {
-        KF_LOAD_A(win_off(g, s));
-        KF_STORE_A(Abuf, s);
-    }
+    KF_LOAD_A(0, win_off(g, 0));
+    KF_LOAD_A(1, win_off(g, 1));
+    KF_STORE_A(Abuf, 0);
+    KF_STORE_A(Abuf, 1);
     if (NTW > 0) KF_LOAD_B(0, win_off(g, 0));
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -254,12 +255,16 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
         for (int s = 0; s < NSUB; ++s) {
             const int gs = ch * NSUB + s;
             // Issue order is pinned (sched_barrier) and every load and store is
-            // unconditional, so the wait counts are exact: each load is consumed one
-            // sub-chunk after it is issued and its latency hides behind these MFMAs.
-            // In the last chunk the parts land in the idle buffer and are never read.
+            // unconditional, so the wait counts are exact.  The next row chunk is loaded
+            // whole at the first sub-chunk and stored after the last, so its latency hides
+            // behind the MFMAs of a full chunk; each column load is consumed one sub-chunk
+            // after it is issued.  In the last chunk the parts land in the idle buffer and
+            // are never read.
             __builtin_amdgcn_sched_barrier(0);
-            if (s > 0) KF_STORE_A(nxt, s - 1);
-            KF_LOAD_A(win_off(g, cn * NSUB + s));
+            if (s == 0) {
+                KF_LOAD_A(0, win_off(g, cn * NSUB));
+                KF_LOAD_A(1, win_off(g, cn * NSUB + 1));
+            }
             if (NTW > 0) KF_LOAD_B((s + 1) % NSUB, win_off(g, min(gs + 1, nsub - 1)));
             __builtin_amdgcn_sched_barrier(0);
             if (NTW == 0) continue;
@@ -283,7 +288,8 @@ __device__ void gram_tiles(double *__restrict__ Abuf, const GramArgs &g, v4d (&c
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        KF_STORE_A(nxt, NSUB - 1);
+        KF_STORE_A(nxt, 0);
+        KF_STORE_A(nxt, 1);
         __syncthreads();
     }
 #undef KF_LOAD_A
