@@ -161,3 +161,20 @@ def test_potrs_repeated_on_same_buffers(prec):
         capi.rfp_potrs(p, A2.ptr, n2, B2.ptr, 1)
         capi.synchronize()
         assert rel_err(B2.numpy(), np.linalg.solve(K2, y2)) < tol, rep
+
+
+@pytest.mark.parametrize("n", [63, 300, 1027])
+def test_potrs_several_right_hand_sides(n):
+    """kf_rfp_potrs with nrhs = 3 (right-hand sides stored one after another)."""
+    from kernelforge_amd import capi
+
+    K = spd(n, 9, shift=1.5)
+    Y = np.random.default_rng(4).standard_normal((3, n))
+    A = capi.DeviceArray.from_numpy(to_rfp(K))
+    B = capi.DeviceArray.from_numpy(Y.copy())
+    capi.rfp_potrf(capi.KF_PREC_F64, A.ptr, n, 0.0)
+    capi.rfp_potrs(capi.KF_PREC_F64, A.ptr, n, B.ptr, 3)
+    capi.synchronize()
+    X = B.numpy().reshape(3, n)
+    for q in range(3):
+        assert rel_err(X[q], np.linalg.solve(K, Y[q])) < 1e-10, q
