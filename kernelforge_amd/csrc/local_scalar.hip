@@ -338,6 +338,44 @@ __device__ __forceinline__ double scan_step(double v, bool take) {
 // The molecules of each set may be reordered (molecules that use the same descriptor
 // windows sorted together, so that blocks share their zero windows): the kernel works in
 // that order and perm maps a molecule back to its row / column of the output.
+// This lane's column of a Gram tile in the epilogue.
+struct TileCol {
+    bool ok;      // a column of the tile
+    double n2;    // |x|^2 of its row of the column set
+    int mol;      // its molecule (block-relative)
+    int seg;      // its window segment
+    int dist;     // distance to the segment's first column
+    bool tail;    // last column of its segment
+};
+
+// Epilogue of one 64 x 64 Gram tile c (this wave's 16 columns): exp of every entry, doubled
+// where DS and the row and column atoms are of one molecule (a sub-tile of a diagonal block
+// whose mirror is not visited), then the segmented scan along the columns; the segment tails
+// go to P[row][segment].  Every lane computes and stores (the entries outside the tile are
+// masked, and lanes that hold no segment tail store to the spare last slot of their row,
+// which no segment uses), so the 16 entries carry no branches and their LDS reads overlap.
+template <bool DS>
+__device__ __forceinline__ void tile_epilogue(const v4d (&c)[4], int lane, int rn, const TileCol &tc,
+                                              const double *n2r, const int *rowmol,
+                                              const double *etab, double cy, double *P) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = x * 16 + mfma16_row_f64(lane, v);
+            const bool ok = tc.ok && row < rn;
+            double e = exp_scaled((n2r[row] + tc.n2 - 2.0 * c[x][v]) * cy, etab);
+            if (DS && rowmol[row] == tc.mol) e *= 2.0;
+            e = ok ? e : 0.0;
+            e = scan_step<1>(e, tc.dist >= 1);
+            e = scan_step<2>(e, tc.dist >= 2);
+            e = scan_step<4>(e, tc.dist >= 4);
+            e = scan_step<8>(e, tc.dist >= 8);
+            P[row * PP + ((ok && tc.tail) ? tc.seg : PP - 1)] = e;
+        }
+    }
+}
+
 struct WinArgs {
     const ulonglong2 *tm1, *tm2;  // tile window masks (null: every window active)
     const int *tb1, *tb2;         // first tile mask of (label, block), l * nblk + block
@@ -350,7 +388,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
     const int *__restrict__ src1, int ma1, const double *__restrict__ Xp2,
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
-    int ma2, int L, int nm1, int nm2, int rep_pad, double cexp, const int2 *__restrict__ order,
+    int ma2, int L, int nm1, int nm2, int rep_pad, double cy, const int2 *__restrict__ order,
     const WinArgs wa, double *__restrict__ out) {
     // two row-operand buffers; the segment partial sums of the epilogue reuse the same
     // memory once the Gram products are done
@@ -362,6 +400,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     __shared__ int rowmol[TS], colmol[2][TS], colseg[2][TS], colinfo[2][TS];
     __shared__ int segf[2][MB], segl[2][MB], rstart[TS + 1], nrr_s;
     __shared__ int mrA[MB + 1], mrB[MB + 1];
+    __shared__ double etab[64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // work-group -> output block through the host-built order: the hardware deals
@@ -381,6 +420,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     ga.r16 = lane & 15;
     ga.kq = lane >> 4;
     for (int k = tid; k < MB * AP; k += NT) accL[k] = 0.0;
+    if (tid < 64) etab[tid] = kExp2Tab64[tid];  // ordered by the label loop's first barrier
 
     for (int l = 0; l < L; ++l) {
         __syncthreads();
@@ -489,30 +529,13 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     if (u == 1) __syncthreads();  // tile 0's row sums are done with P
                     {
                         const int col = bcol;
-                        const bool colok = col < cn[u];
-                        const double nc = n2c[u][col];
-                        const int cm = colmol[u][col], seg = colseg[u][col];
-                        const int info = colinfo[u][col], dist = info & 255;
-                        const bool stail = (info >> 8) & 1;
-#pragma unroll
-                        for (int x = 0; x < 4; ++x) {
-                            const v4d cv = c[u][x];
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) {
-                                const int row = x * 16 + mfma16_row_f64(lane, v);
-                                const bool ok = colok && row < rn;
-                                double e = 0.0;
-                                if (ok) {
-                                    e = exp((n2r[row] + nc - 2.0 * cv[v]) * cexp);
-                                    if (dbl_self && rowmol[row] == cm) e *= 2.0;
-                                }
-                                e = scan_step<1>(e, dist >= 1);
-                                e = scan_step<2>(e, dist >= 2);
-                                e = scan_step<4>(e, dist >= 4);
-                                e = scan_step<8>(e, dist >= 8);
-                                if (ok && stail) P[row * PP + seg] = e;
-                            }
-                        }
+                        const int info = colinfo[u][col];
+                        const TileCol tc{col < cn[u], n2c[u][col], colmol[u][col], colseg[u][col],
+                                         info & 255, ((info >> 8) & 1) != 0};
+                        if (dbl_self)
+                            tile_epilogue<true>(c[u], lane, rn, tc, n2r, rowmol, etab, cy, P);
+                        else
+                            tile_epilogue<false>(c[u], lane, rn, tc, n2r, rowmol, etab, cy, P);
                     }
                     __syncthreads();
                     // thread (wave w, lane b) adds, for column molecule b, every 4th row
@@ -791,7 +814,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     DevicePacked d;
     if ((rc = upload_and_pack(p, perm, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
     const int nblk = (nm + MB - 1) / MB;
-    const double cexp = -1.0 / (2.0 * sigma * sigma);
+    const double cy = -kExpScale / (2.0 * sigma * sigma);  // squared distance -> exp_scaled's y
     if (p.nlabels == 0) {  // no atoms at all: zero output
         const size_t n = MODE == SYMM_RFP ? (size_t)nm * (nm + 1) / 2 : (size_t)nm * nm;
         KF_HIP_TRY(hipMemsetAsync(out, 0, n * sizeof(double), stream));
@@ -803,7 +826,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3(grid), dim3(NT), 0, stream,
                        d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
-                       p.nlabels, nm, nm, d.rep_pad, cexp, order, win_args(d, d),
+                       p.nlabels, nm, nm, d.rep_pad, cy, order, win_args(d, d),
                        static_cast<double *>(out));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
@@ -879,7 +902,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     if ((rc = upload_and_pack(p1, perm1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
     if ((rc = upload_and_pack(p2, perm2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
     const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
-    const double cexp = -1.0 / (2.0 * sigma * sigma);
+    const double cy = -kExpScale / (2.0 * sigma * sigma);  // squared distance -> exp_scaled's y
     const int2 *order = nullptr;
     unsigned grid = 0;
     if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order, &grid))) return rc;
@@ -887,7 +910,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3(grid),
                        dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
                        d2.n2, d2.mol_row, d2.src, max_atoms2, (int)labels.size(), nm1, nm2,
-                       d1.rep_pad, cexp, order, win_args(d1, d2), static_cast<double *>(K));
+                       d1.rep_pad, cy, order, win_args(d1, d2), static_cast<double *>(K));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
