@@ -316,14 +316,15 @@ __device__ __forceinline__ void gram_dispatch(double *Abuf, const GramArgs &g, i
     }
 }
 
-// v + (value of the lane D places lower in the same 16-lane row), when `take`: one step
-// of a segmented inclusive scan along the 16 columns a row of MFMA output spans.
+// v + take * (value of the lane D places lower in the same 16-lane row), take = 0 or 1:
+// one step of a segmented inclusive scan along the 16 columns a row of MFMA output spans
+// (exact: the product is the shifted value or zero).
 template <int D>
-__device__ __forceinline__ double scan_step(double v, bool take) {
+__device__ __forceinline__ double scan_step(double v, double take) {
     const int lo = __double2loint(v), hi = __double2hiint(v);
     const int slo = __builtin_amdgcn_update_dpp(0, lo, 0x110 + D, 0xF, 0xF, true);
     const int shi = __builtin_amdgcn_update_dpp(0, hi, 0x110 + D, 0xF, 0xF, true);
-    return take ? v + __hiloint2double(shi, slo) : v;
+    return fma(__hiloint2double(shi, slo), take, v);
 }
 
 // Output block: 64 x 64 molecule pairs per work-group, accumulated in LDS (each entry
@@ -340,38 +341,37 @@ __device__ __forceinline__ double scan_step(double v, bool take) {
 // that order and perm maps a molecule back to its row / column of the output.
 // This lane's column of a Gram tile in the epilogue.
 struct TileCol {
-    bool ok;      // a column of the tile
-    double n2;    // |x|^2 of its row of the column set
-    int mol;      // its molecule (block-relative)
-    int seg;      // its window segment
-    int dist;     // distance to the segment's first column
-    bool tail;    // last column of its segment
+    double n2;                  // cy * |x|^2 of its row of the column set
+    double t1, t2, t4, t8;      // 1 where the scan step of that distance stays in the segment
+    int mol;                    // its molecule (block-relative)
+    int slot;                   // its segment if it is the segment's last column, else PP - 1
 };
 
 // Epilogue of one 64 x 64 Gram tile c (this wave's 16 columns): exp of every entry, doubled
 // where DS and the row and column atoms are of one molecule (a sub-tile of a diagonal block
 // whose mirror is not visited), then the segmented scan along the columns; the segment tails
-// go to P[row][segment].  Every lane computes and stores (the entries outside the tile are
-// masked, and lanes that hold no segment tail store to the spare last slot of their row,
-// which no segment uses), so the 16 entries carry no branches and their LDS reads overlap.
+// go to P[row][segment].  No entry is masked or branched on: rows past the tile are never
+// read back, columns past it are the last lanes of a row and never feed a segment of the
+// tile (a scan only takes from lower lanes of its own segment), and lanes that hold no
+// segment tail store to the spare last slot of their row, which no segment uses.  n2r holds
+// cy * |x|^2 of the rows.
 template <bool DS>
-__device__ __forceinline__ void tile_epilogue(const v4d (&c)[4], int lane, int rn, const TileCol &tc,
+__device__ __forceinline__ void tile_epilogue(const v4d (&c)[4], int lane, const TileCol &tc,
                                               const double *n2r, const int *rowmol,
-                                              const double *etab, double cy, double *P) {
+                                              const double *etab, double m2cy, double *P) {
+    double *const pl = P + (lane >> 4) * PP + tc.slot;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = x * 16 + mfma16_row_f64(lane, v);
-            const bool ok = tc.ok && row < rn;
-            double e = exp_scaled((n2r[row] + tc.n2 - 2.0 * c[x][v]) * cy, etab);
+            double e = exp_scaled(fma(c[x][v], m2cy, n2r[row] + tc.n2), etab);
             if (DS && rowmol[row] == tc.mol) e *= 2.0;
-            e = ok ? e : 0.0;
-            e = scan_step<1>(e, tc.dist >= 1);
-            e = scan_step<2>(e, tc.dist >= 2);
-            e = scan_step<4>(e, tc.dist >= 4);
-            e = scan_step<8>(e, tc.dist >= 8);
-            P[row * PP + ((ok && tc.tail) ? tc.seg : PP - 1)] = e;
+            e = scan_step<1>(e, tc.t1);
+            e = scan_step<2>(e, tc.t2);
+            e = scan_step<4>(e, tc.t4);
+            e = scan_step<8>(e, tc.t8);
+            pl[(x * 16 + 4 * v) * PP] = e;  // row x*16 + 4v + lane/16
         }
     }
 }
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     const bool v = lane < rn;
                     const int m = v ? src1[r0 + lane] / ma1 - a0 : -1;
                     rowmol[lane] = m;
-                    n2r[lane] = v ? n2a[r0 + lane] : 0.0;
+                    n2r[lane] = v ? cy * n2a[r0 + lane] : 0.0;
                     const int prev = __shfl_up(m, 1, 64);
                     const bool rh = v && (lane == 0 || prev != m);
                     const unsigned long long rheads = __ballot(rh);
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     const bool v = lane < cn[u];
                     segf[u][lane] = -1;  // same wave: ordered before the writes below
                     const int m = v ? src2[c0[u] + lane] / ma2 - b0 : -1;
-                    n2c[u][lane] = v ? n2b[c0[u] + lane] : 0.0;
+                    n2c[u][lane] = v ? cy * n2b[c0[u] + lane] : 0.0;
                     const int prev = __shfl_up(m, 1, 64), next = __shfl_down(m, 1, 64);
                     const bool mhead = v && (lane == 0 || prev != m);
                     const bool mtail = v && (lane == 63 || next != m);
@@ -529,13 +529,16 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
                     if (u == 1) __syncthreads();  // tile 0's row sums are done with P
                     {
                         const int col = bcol;
-                        const int info = colinfo[u][col];
-                        const TileCol tc{col < cn[u], n2c[u][col], colmol[u][col], colseg[u][col],
-                                         info & 255, ((info >> 8) & 1) != 0};
+                        const int info = colinfo[u][col], dist = info & 255;
+                        const bool tail = col < cn[u] && ((info >> 8) & 1);
+                        const TileCol tc{n2c[u][col],
+                                         dist >= 1 ? 1.0 : 0.0, dist >= 2 ? 1.0 : 0.0,
+                                         dist >= 4 ? 1.0 : 0.0, dist >= 8 ? 1.0 : 0.0,
+                                         colmol[u][col], tail ? colseg[u][col] : PP - 1};
                         if (dbl_self)
-                            tile_epilogue<true>(c[u], lane, rn, tc, n2r, rowmol, etab, cy, P);
+                            tile_epilogue<true>(c[u], lane, tc, n2r, rowmol, etab, -2.0 * cy, P);
                         else
-                            tile_epilogue<false>(c[u], lane, rn, tc, n2r, rowmol, etab, cy, P);
+                            tile_epilogue<false>(c[u], lane, tc, n2r, rowmol, etab, -2.0 * cy, P);
                     }
                     __syncthreads();
                     // thread (wave w, lane b) adds, for column molecule b, every 4th row
