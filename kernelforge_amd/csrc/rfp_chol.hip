@@ -155,9 +155,12 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
 // Lane i keeps row i of the tile in registers.  Cholesky, right-looking: per column j
 // the pivot comes from lane j (readlane), every lane scales its L[i][j] and publishes
 // it in LDS, and updates its row with the column read back (broadcast reads; one
-// wave, so LDS order needs no barrier).  Inverse: lane c keeps column c of L^-1 and
-// runs forward substitution over the rows of L (broadcast reads).  A partial tile
-// (w < 64) is padded with the identity, which keeps both factors block diagonal.
+// wave, so LDS order needs no barrier).  Inverse, blocked on the two 32-row halves,
+//   L = [[L0, 0], [B, L1]],  L^-1 = [[L0^-1, 0], [-L1^-1 B L0^-1, L1^-1]]:
+// lane i runs forward substitution for column i % 32 of the inverse of diagonal block
+// i / 32 (both blocks at once, 32 steps), and the off-diagonal block is two 32x32x32
+// MFMA products.  A partial tile (w < 64) is padded with the identity, which keeps both
+// factors block diagonal.
 template <typename T>
 __device__ __forceinline__ T lane_value(T v, int l);
 template <>
@@ -170,12 +173,19 @@ template <>
 __device__ __forceinline__ float lane_value<float>(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+__device__ __forceinline__ double recip_sqrt(double v) { return rsqrt(v); }
+__device__ __forceinline__ float recip_sqrt(float v) { return rsqrtf(v); }
+
+constexpr int HB = NB / 2;   // half block of the blocked inverse
+constexpr int HP = HB + 1;   // LDS pitch of its pieces
 
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
     __shared__ T S[NB * TP];
     __shared__ T col[NB];
     __shared__ T dinv[NB];
+    __shared__ T Di[NB * HP];  // row 32h + q, column c: (L_h^-1)[q][c]
+    __shared__ T Zs[HB * HP];  // B L0^-1
     if (*info) return;
     // this wave is the whole critical path of the factorization: let it issue ahead of the
     // trailing-update waves that share its CU
@@ -198,8 +208,8 @@ __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, in
             fail = j + 1;
             break;
         }
-        const T d = sqrt(piv), rd = T(1) / d;
-        r[j] = i == j ? d : r[j] * rd;
+        const T rd = recip_sqrt(piv);
+        r[j] = i == j ? piv * rd : r[j] * rd;
         col[i] = r[j];  // read back below by the same wave: LDS keeps its order
         if (i == 0) dinv[j] = rd;
         // all of column j first, then the updates: one wait for the whole batch
@@ -218,23 +228,62 @@ __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, in
     for (int c = 0; c < NB; ++c) S[i * TP + c] = c <= i ? r[c] : T(0);
     __syncthreads();
     store_tile<T, true>(a, t, w, w, S);
-    // column c = i of L^-1: x[q] = (delta_qc - sum_{k<q} L[q][k] x[k]) / L[q][q], with the
-    // reciprocals of the diagonal kept from the factorization
-    T x[NB];
+    // column cc of L_h^-1 (h = i / 32): x[q] = (delta_q,cc - sum_{k<q} L_h[q][k] x[k]) / L_h[q][q],
+    // with the reciprocals of the diagonal kept from the factorization
+    const int h = i >> 5, cc = i & (HB - 1);
+    {
+        T x[HB];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-        T lq[NB];
+        for (int q = 0; q < HB; ++q) {
+            const int g = h * HB + q;
+            T lq[HB];
 #pragma unroll
-        for (int k = 0; k < q; ++k) lq[k] = S[q * TP + k];
-        const T dq = dinv[q];
-        __builtin_amdgcn_sched_barrier(0);
-        T acc[4] = {q == i ? T(1) : T(0), T(0), T(0), T(0)};
+            for (int k = 0; k < q; ++k) lq[k] = S[g * TP + h * HB + k];
+            const T dq = dinv[g];
+            __builtin_amdgcn_sched_barrier(0);
+            T acc[4] = {q == cc ? T(1) : T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int k = 0; k < q; ++k) acc[k & 3] -= lq[k] * x[k];
-        x[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * dq;
+            for (int k = 0; k < q; ++k) acc[k & 3] -= lq[k] * x[k];
+            x[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * dq;
+        }
+#pragma unroll
+        for (int q = 0; q < HB; ++q) Di[(h * HB + q) * HP + cc] = x[q];
+    }
+    // Z = B L0^-1, then W = -L1^-1 Z, as 16x16x4 MFMA fragments (A[m][k] from lane
+    // m + 16k, B[k][n] from lane n + 16k), one 16x16 fragment at a time (the Cholesky above
+    // holds the register file); one wave, so the LDS order holds
+    typedef typename Vec4<T>::type V4;
+    const int l16 = i & 15, kq = i >> 4;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int u = f >> 1, v = f & 1;
+        V4 z = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < HB; k0 += 4)
+            z = mfma4(S[(HB + u * 16 + l16) * TP + k0 + kq], Di[(k0 + kq) * HP + v * 16 + l16], z);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Zs[(u * 16 + mfma16_row<T>(i, e)) * HP + v * 16 + l16] = z[e];
+    }
+    // inv (row-major, NB x NB): the lower-left block W, the diagonal blocks from Di, zeros above
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int u = f >> 1, v = f & 1;
+        V4 z = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < HB; k0 += 4)
+            z = mfma4(-Di[(HB + u * 16 + l16) * HP + k0 + kq], Zs[(k0 + kq) * HP + v * 16 + l16], z);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int q = HB + u * 16 + mfma16_row<T>(i, e), c = v * 16 + l16;
+            inv[q * NB + c] = (q < w && c < w) ? z[e] : T(0);
+        }
     }
 #pragma unroll
-    for (int q = 0; q < NB; ++q) inv[q * NB + i] = (q < w && i < w) ? x[q] : T(0);
+    for (int q = 0; q < NB; ++q) {
+        const int hq = q / HB;
+        if (hq == h) inv[q * NB + i] = (q < w && i < w) ? Di[q * HP + cc] : T(0);
+        else if (hq == 0) inv[q * NB + i] = T(0);  // above the diagonal blocks
+    }
 }
 
 // Panel: A_IJ <- A_IJ * inv^T for every block row I > J (one WG per I).
