@@ -450,8 +450,7 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
 // Chunk depth of the trailing update.  8 keeps two work-groups at 36 KB of LDS each, so a
 // CU running two of them still has room for a panel-factorization kernel (the critical
 // path of the look-ahead) instead of holding it back until one of them retires.
-#define KF_SYRK_SK 8
-constexpr int SK = KF_SYRK_SK;
+constexpr int SK = 8;
 constexpr int SP = SK + 1;   // LDS pitch
 
 struct PairList {  // pairs of blocks from Kbeg: lower region first, then upper
@@ -520,11 +519,11 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
     // columns tid / 64 + 4e, row-major blocks column tid % SK and rows tid / SK + RS e
     constexpr int SE = NB * SK / 256, RS = 256 / SK;
     const int rc = tid & 63, cc_c = tid >> 6, rr_r = tid / SK, cc_r = tid % SK;
-    T st[4][SE];
-    // one staged block b of chunk kc of panel column block J: the loads are unconditional
+    T st[2][4][SE];  // two chunks in flight: chunk c is staged in slot c & 1
+    // one staged block b of chunk kc of panel column block J into slot sl: the loads are unconditional
     // (clamped addresses); columns past the block's width are zeroed when the values are
     // stored to LDS, so nothing waits on a load until then
-#define KF_SYRK_LOAD(b, J, kc)                                                                    \
+#define KF_SYRK_LOAD(sl, b, J, kc)                                                                \
     do {                                                                                          \
         const int c0_ = P.start(J), wl_ = P.width(J) - 1;                                         \
         constexpr bool cm_ = (b) < 2 ? CMI : CMK;                                                 \
@@ -532,77 +531,90 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
             const int r_ = cm_ ? rc : rr_r + RS * e;                                              \
             const int c_ = (kc) * SK + (cm_ ? cc_c + 4 * e : cc_r);                               \
             const int rcl = min(r_, rmax[b]), ccl = c0_ + min(c_, wl_);                           \
-            st[b][e] = cm_ ? src[b][rcl + (size_t)ccl * s] : src[b][(size_t)rcl * s + ccl];       \
+            st[sl][b][e] = cm_ ? src[b][rcl + (size_t)ccl * s] : src[b][(size_t)rcl * s + ccl];   \
         }                                                                                         \
     } while (0)
-#define KF_SYRK_STORE(buf, b, J, kc)                                                              \
+#define KF_SYRK_STORE(buf, sl, b, J, kc)                                                          \
     do {                                                                                          \
         const int wl_ = P.width(J) - 1;                                                           \
         constexpr bool cm_ = (b) < 2 ? CMI : CMK;                                                 \
         _Pragma("unroll") for (int e = 0; e < SE; ++e) {                                          \
             const int r_ = cm_ ? rc : rr_r + RS * e;                                              \
             const int c_ = cm_ ? cc_c + 4 * e : cc_r;                                             \
-            (buf)[((b) * NB + r_) * SP + c_] = (kc) * SK + c_ <= wl_ ? st[b][e] : T(0);          \
+            (buf)[((b) * NB + r_) * SP + c_] = (kc) * SK + c_ <= wl_ ? st[sl][b][e] : T(0);      \
         }                                                                                         \
     } while (0)
 #define KF_SYRK_ALL(X) X(0); X(1); X(2); X(3)
     const int nch = (Jhi - Jlo) * (NB / SK);
-#define KF_L0(b) KF_SYRK_LOAD(b, Jlo, 0)
-#define KF_L1(b) KF_SYRK_LOAD(b, Jlo, 1)
-#define KF_S0(b) KF_SYRK_STORE(Lb, b, Jlo, 0)
+    // prologue: chunk 0 to LDS, chunks 1 and 2 in flight (clamped to the last chunk, which
+    // is then staged again into a buffer nobody reads: every load and store is unconditional)
+    const int p1 = min(1, nch - 1), p2 = min(2, nch - 1);
+#define KF_L0(b) KF_SYRK_LOAD(0, b, Jlo, 0)
+#define KF_S0(b) KF_SYRK_STORE(Lb, 0, b, Jlo, 0)
+#define KF_L1(b) KF_SYRK_LOAD(1, b, Jlo + p1 / (NB / SK), p1 % (NB / SK))
+#define KF_L2(b) KF_SYRK_LOAD(0, b, Jlo + p2 / (NB / SK), p2 % (NB / SK))
     KF_SYRK_ALL(KF_L0);
     KF_SYRK_ALL(KF_S0);
-    if (nch > 1) { KF_SYRK_ALL(KF_L1); }
+    KF_SYRK_ALL(KF_L1);
+    KF_SYRK_ALL(KF_L2);
 #undef KF_L0
-#undef KF_L1
 #undef KF_S0
+#undef KF_L1
+#undef KF_L2
     __syncthreads();
     // rows of the computed tile (X) and its columns (Y) inside a buffer
     const int xo = trans ? (2 + wj) * NB : wi * NB, yo = trans ? wi * NB : (2 + wj) * NB;
-    for (int ch = 0; ch < nch; ++ch) {
-        const T *cur = Lb + (ch & 1) * (4 * NB * SP);
-        T *nxt = Lb + ((ch + 1) & 1) * (4 * NB * SP);
-        // the last chunk re-stages itself into the idle buffer (never read): every load and
-        // store is unconditional, so the wait counts stay exact across iterations
-        const int c1 = min(ch + 1, nch - 1), c2 = min(ch + 2, nch - 1);
-        const int J1 = Jlo + c1 / (NB / SK), kc1 = c1 % (NB / SK);
-        const int J2 = Jlo + c2 / (NB / SK), kc2 = c2 % (NB / SK);
-        // k-step ks of this chunk, then staged blocks of the next chunk moved to LDS and the
-        // same blocks of the chunk after it fetched: the staging spreads over the MFMAs
-#define KF_SYRK_MMA(ks)                                                                           \
+    // one chunk ch whose successor sits in register slot SL = (ch + 1) & 1 (a constant: the
+    // loop below takes two chunks per trip, and nch is a multiple of NB / SK = 8)
+    // the k-steps keep their branch on `active`: without it the compiler overlaps the next
+    // k-step's LDS reads with the MFMAs at 246 VGPRs, and the diagonal-tile kernel of the
+    // panel chain (296) then no longer fits beside a trailing-update wave on a SIMD
+#define KF_SYRK_MMA(cur, ks)                                                                      \
         do {                                                                                      \
             if (active) {                                                                         \
                 const int kk = 4 * (ks) + (lane >> 4);                                            \
                 T av[4], bv[4];                                                                   \
                 _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
-                    av[i] = cur[(xo + i * 16 + (lane & 15)) * SP + kk];                           \
+                    av[i] = (cur)[(xo + i * 16 + (lane & 15)) * SP + kk];                         \
                 _Pragma("unroll") for (int j = 0; j < 4; ++j)                                     \
-                    bv[j] = cur[(yo + j * 16 + (lane & 15)) * SP + kk];                           \
+                    bv[j] = (cur)[(yo + j * 16 + (lane & 15)) * SP + kk];                         \
                 _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
                     _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
                         acc[i][j] = mfma4(-av[i], bv[j], acc[i][j]);                              \
             }                                                                                     \
         } while (0)
-#define KF_SYRK_STAGE(b)                                                                          \
+#define KF_SYRK_STAGE(nxt, SL, b, J1, kc1, J3, kc3)                                               \
         do {                                                                                      \
-            KF_SYRK_STORE(nxt, b, J1, kc1);                                                       \
-            KF_SYRK_LOAD(b, J2, kc2);                                                             \
+            KF_SYRK_STORE(nxt, SL, b, J1, kc1);                                                   \
+            KF_SYRK_LOAD(SL, b, J3, kc3);                                                         \
         } while (0)
-#if KF_SYRK_SK == 16
-        KF_SYRK_MMA(0); KF_SYRK_STAGE(0);
-        KF_SYRK_MMA(1); KF_SYRK_STAGE(1);
-        KF_SYRK_MMA(2); KF_SYRK_STAGE(2);
-        KF_SYRK_MMA(3); KF_SYRK_STAGE(3);
-#elif KF_SYRK_SK == 8
-        KF_SYRK_MMA(0); KF_SYRK_STAGE(0); KF_SYRK_STAGE(1);
-        KF_SYRK_MMA(1); KF_SYRK_STAGE(2); KF_SYRK_STAGE(3);
-#else
-#error "trailing-update chunk depth must be 8 or 16"
-#endif
+    // the last chunks re-stage the last one into the idle buffer (never read): every load
+    // and store is unconditional, so the wait counts stay exact across iterations.  Each
+    // k-step is followed by staged blocks of the next chunk moved to LDS and the same
+    // blocks of the chunk three ahead fetched into the freed slot: the staging spreads
+    // over the MFMAs and each load has two chunks of MFMAs to arrive.
+#define KF_SYRK_ITER(ch, SL)                                                                      \
+    do {                                                                                          \
+        const T *cur_ = Lb + ((ch) & 1) * (4 * NB * SP);                                          \
+        T *nxt_ = Lb + (((ch) + 1) & 1) * (4 * NB * SP);                                          \
+        const int c1_ = min((ch) + 1, nch - 1), c3_ = min((ch) + 3, nch - 1);                     \
+        const int J1_ = Jlo + c1_ / (NB / SK), kc1_ = c1_ % (NB / SK);                            \
+        const int J3_ = Jlo + c3_ / (NB / SK), kc3_ = c3_ % (NB / SK);                            \
+        KF_SYRK_MMA(cur_, 0);                                                                     \
+        KF_SYRK_STAGE(nxt_, SL, 0, J1_, kc1_, J3_, kc3_);                                         \
+        KF_SYRK_STAGE(nxt_, SL, 1, J1_, kc1_, J3_, kc3_);                                         \
+        KF_SYRK_MMA(cur_, 1);                                                                     \
+        KF_SYRK_STAGE(nxt_, SL, 2, J1_, kc1_, J3_, kc3_);                                         \
+        KF_SYRK_STAGE(nxt_, SL, 3, J1_, kc1_, J3_, kc3_);                                         \
+        __syncthreads();                                                                          \
+    } while (0)
+    for (int ch = 0; ch < nch; ch += 2) {
+        KF_SYRK_ITER(ch, 1);
+        KF_SYRK_ITER(ch + 1, 0);
+    }
+#undef KF_SYRK_ITER
 #undef KF_SYRK_MMA
 #undef KF_SYRK_STAGE
-        __syncthreads();
-    }
 #undef KF_SYRK_ALL
 #undef KF_SYRK_LOAD
 #undef KF_SYRK_STORE
