@@ -96,6 +96,31 @@ def test_config0_descriptors(lk, config0):
     assert rel_err(Kr, oracle.kernel_rect(X[800:], X[:800], Q[800:], Q[:800], N[800:], N[:800], 2.0)) < 1e-10
 
 
+@pytest.mark.parametrize("sigma", [4.0, 1.0, 0.5, 0.25])
+def test_exp_range_exact_distances(lk, sigma):
+    """One atom per molecule on an integer grid t = 0..24: every squared distance
+    (t_a - t_b)^2 and 1/(2 sigma^2) is exact, so each entry is exp of an exact argument,
+    from 0 down to -4608 (through the subnormal range and past underflow).  Checked entry
+    by entry against np.exp; the tolerance covers the Gram form's cancellation
+    (|x_a|^2 + |x_b|^2 - 2 x_a.x_b, rounded in units of t_a^2 + t_b^2) and the argument's
+    rounding, a few ulp per unit of |argument|, not the exp: a wrong table entry or a wrong
+    polynomial coefficient is far outside it."""
+    t = np.random.default_rng(5).permutation(25).astype(np.float64)
+    x = t.reshape(-1, 1, 1)
+    q = np.ones((25, 1), np.int32)
+    n = np.ones(25, np.int32)
+    c = 1.0 / (2.0 * sigma * sigma)
+    d2 = (t[:, None] - t[None, :]) ** 2
+    ref = np.exp(-d2 * c)
+    tol = 2.3e-16 * (4.0 + 4.0 * d2 * c + 4.0 * (t[:, None] ** 2 + t[None, :] ** 2) * c)
+    for K in (lk.kernel_gaussian_symm(x, q, n, sigma),
+              lk.kernel_gaussian(x[:11], x, q[:11], q, n[:11], n, sigma)):
+        r = ref[: K.shape[0]]
+        normal = r > 1e-300
+        assert np.all(np.abs(K - r)[normal] <= (tol[: K.shape[0]] * r)[normal])
+        assert np.all(np.abs(K - r)[~normal] <= 1e-300)
+
+
 def test_no_shared_labels_and_clamping(lk):
     x, q, n = random_set(9, 4, 8, 2, 3)
     q2 = q + 10
