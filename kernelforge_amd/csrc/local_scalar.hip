@@ -625,9 +625,11 @@ std::vector<int> rank_windows(const std::vector<const std::vector<unsigned long 
                               int nwin) {
     std::vector<long> freq(nwin, 0);
     for (const auto *m : sets)
-        for (size_t a = 0; 2 * a < m->size(); ++a)
-            for (int w = 0; w < nwin; ++w)
-                if (((*m)[2 * a + (w >> 6)] >> (w & 63)) & 1ull) ++freq[w];
+        for (size_t e = 0; e < m->size(); ++e)  // set bits only: word e & 1 holds windows 64 (e & 1) + ..
+            for (unsigned long long bits = (*m)[e]; bits; bits &= bits - 1) {
+                const int w = 64 * (int)(e & 1) + __builtin_ctzll(bits);
+                if (w < nwin) ++freq[w];
+            }
     std::vector<int> rank(nwin);
     for (int w = 0; w < nwin; ++w) rank[w] = w;
     std::stable_sort(rank.begin(), rank.end(), [&](int x, int y) { return freq[x] < freq[y]; });
@@ -640,18 +642,23 @@ std::vector<int> rank_windows(const std::vector<const std::vector<unsigned long 
 std::vector<int> order_molecules(const std::vector<unsigned long long> &mask,
                                  const std::vector<int> &rank) {
     const int nm = (int)(mask.size() / 2), nk = std::min<int>(64, (int)rank.size());
-    std::vector<unsigned long long> key(nm, 0ull);
+    // bit of window w in the key: rarest (rank position 0) most significant
+    int shift[2 * 64];
+    std::fill(shift, shift + 2 * 64, -1);
+    for (int t = 0; t < nk; ++t) shift[rank[t]] = nk - 1 - t;
+    std::vector<std::pair<unsigned long long, int>> key(nm);
     for (int a = 0; a < nm; ++a) {
         unsigned long long k = 0;
-        for (int t = 0; t < nk; ++t) {
-            const int w = rank[t];
-            k = (k << 1) | ((mask[2 * (size_t)a + (w >> 6)] >> (w & 63)) & 1ull);
-        }
-        key[a] = k;
+        for (int h = 0; h < 2; ++h)
+            for (unsigned long long bits = mask[2 * (size_t)a + h]; bits; bits &= bits - 1) {
+                const int sh = shift[64 * h + __builtin_ctzll(bits)];
+                if (sh >= 0) k |= 1ull << sh;
+            }
+        key[a] = {k, a};
     }
+    std::sort(key.begin(), key.end());  // (key, input index): ties stay in input order
     std::vector<int> perm(nm);
-    for (int a = 0; a < nm; ++a) perm[a] = a;
-    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+    for (int a = 0; a < nm; ++a) perm[a] = key[a].second;
     return perm;
 }
 
