@@ -403,8 +403,10 @@ __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) 
 // For one atom i of x the columns are every label-matched (k, j) of the group:
 //   V_i (3 n_x x ncol) = J_xi^T (3 n_x x rep) * D (rep x ncol),  D[f][c] = x_xi[f] - x_yj[f]
 // which streams each J_xi row once for the whole group (the one-pair form streamed it
-// once per pair).  Wave w takes column fragments w, w+4, ...; A operands (J rows) and B
-// operands (differences, formed in registers) come straight from HBM, 16 features ahead.
+// once per pair).  The columns depend only on the label of atom i, so they are listed
+// once per label of x, and the (atom, 16-column fragment) items of a label are dealt
+// over the four waves; A operands (J rows) and B operands (differences, formed in
+// registers) come straight from HBM, 16 features ahead.
 // ---------------------------------------------------------------------------
 constexpr int PG = 8;  // partners per group
 
@@ -417,7 +419,7 @@ struct ProjGroup {
 template <int NPF>
 __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const ProjGroup *groups) {
     __shared__ int qx[MAXA], qy[PG][MAXA], ny[PG], ist[PG][MAXA], rnk[PG][MAXA], rx[MAXA];
-    __shared__ int cols[PG * MAXA], wbase[5], gy[PG], gslot[PG];
+    __shared__ int cols[PG * MAXA], wbase[5], gy[PG], gslot[PG], xl[MAXA], s_nl;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, col = lane & 15, kr = lane >> 4;
     const ProjGroup *gp = groups + blockIdx.x;
     const bool s0 = gp->side == 0;
@@ -474,9 +476,14 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
 #pragma unroll
     for (int y = 0; y < NPF; ++y) pofs[y] = min(16 * y + col, max(ncx, 1) - 1);
 
-    for (int i = 0; i < nx; ++i) {
-        __syncthreads();  // tables ready / the previous atom's columns consumed
-        const int l = qx[i];
+    // one pass per label of x (taken at its first atom, rx == 0): the columns are every
+    // label-matched (k, j) of the group, the same for all atoms of x with that label, and
+    // the (atom, column fragment) items of the label are dealt over the four waves
+    __syncthreads();  // tables ready (rx decides the passes below, uniformly)
+    for (int i0 = 0; i0 < nx; ++i0) {
+        if (rx[i0] != 0) continue;  // not the first atom of its label (uniform)
+        __syncthreads();  // the previous label's columns and atoms consumed
+        const int l = qx[i0];
         {
             // columns: every (k, j) with q_y[k][j] == l, k-major
             const int k = tid >> 5, j = tid & 31;
@@ -484,6 +491,12 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
             const unsigned long long bal = __ballot(v);
             const int pre = __popcll(bal & ((1ull << lane) - 1));
             if (lane == 0) wbase[wv + 1] = __popcll(bal);
+            // atoms of x with label l, in order (rx is the rank among them)
+            if (tid < MAXA && tid < nx && qx[tid] == l) xl[rx[tid]] = tid;
+            if (wv == 0) {
+                const unsigned long long bx = __ballot(tid < MAXA && tid < nx && qx[tid] == l);
+                if (lane == 0) s_nl = __popcll(bx);
+            }
             __syncthreads();
             if (tid == 0) {
                 wbase[0] = 0;
@@ -495,9 +508,12 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
         }
         const int ncol = wbase[4];
         const int ncf = (ncol + 15) >> 4;
+        const int nitems = s_nl * ncf;
+        for (int it = wv; it < nitems; it += 4) {
+        const int i = xl[it / ncf], z = it % ncf;
         const double *xi = xx0 + (size_t)i * rep;
         const double *Ji = Jx0 + (size_t)i * rep * lda;
-        for (int z = wv; z < ncf; z += 4) {
+        {
             const int c = min(16 * z + col, ncol - 1);
             const int kc = cols[c] >> 5, jc = cols[c] & 31;
             const double *xj = Xy + ((size_t)gy[kc] * may + jc) * rep;
@@ -566,6 +582,7 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
                         if (p < ncx) U[p] = -acc[y][v];
                     }
             }
+        }
         }
     }
 }
