@@ -705,7 +705,35 @@ __global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *
     }
 }
 
+// The tile L(I, J) of a step, for block rows/columns of widths rn x cn, into registers
+// (thread t holds elements t + 256 e, coalesced along the contiguous axis; zero outside
+// the tile), so that its loads are in flight while the step forms its diagonal product.
+constexpr int SE_T = NB * NB / 256;
+template <typename T>
+__device__ __forceinline__ void step_tile_load(const T *a, const Tile &t, int rn, int cn,
+                                               double (&v)[SE_T]) {
+#pragma unroll
+    for (int e = 0; e < SE_T; ++e) {
+        const int idx = threadIdx.x + 256 * e;
+        int r, c;
+        if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
+        const double x = (double)a[t.base + (size_t)min(r, rn - 1) * t.rs + (size_t)min(c, cn - 1) * t.cs];
+        v[e] = (r < rn && c < cn) ? x : 0.0;
+    }
+}
+__device__ __forceinline__ void step_tile_store(const Tile &t, const double (&v)[SE_T], double *Lt) {
+#pragma unroll
+    for (int e = 0; e < SE_T; ++e) {
+        const int idx = threadIdx.x + 256 * e;
+        int r, c;
+        if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
+        Lt[r * TP + c] = v[e];
+    }
+}
+
 // forward step J: y_J = Linv_J b_J (stored to y by WG 0); b_I -= L_IJ y_J for I > J.
+// The products run over all 64 columns with the terms past the block width masked, so
+// every load of a product is issued at once.
 template <typename T>
 __global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const double *inv,
                                                 double *b, double *y, int n, int nrhs) {
@@ -715,35 +743,43 @@ __global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const
     const int c0 = P.start(J), w = P.width(J);
     const double *Li = inv + (size_t)J * NB * NB;
     int r0 = 0, rn = 0;
+    Tile t{};
+    double lt[SE_T];
     if (blockIdx.x > 0) {
         r0 = P.start(I);
         rn = P.width(I);
-        const Tile t = tile_at(P, r0, c0);
-        for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-            int r, c;
-            if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-            Lt[r * TP + c] = (r < rn && c < w) ? (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs] : 0.0;
-        }
+        t = tile_at(P, r0, c0);
+        step_tile_load(a, t, rn, w, lt);
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
         // y_J = Linv_J b_J: 4 threads per row
         {
             const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            const double *bq = b + (size_t)q * n + c0;
             double s = 0.0;
-            if (r < w)
-                for (int c = qq; c < w; c += 4) s += Li[r * NB + c] * b[(size_t)q * n + c0 + c];
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) {
+                const int c = qq + 4 * k;
+                const double lv = Li[r * NB + c], bv = bq[min(c, w - 1)];
+                s += c < w ? lv * bv : 0.0;
+            }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && r < NB) yj[r] = r < w ? s : 0.0;
         }
+        if (q == 0 && blockIdx.x > 0) step_tile_store(t, lt, Lt);
         __syncthreads();
         if (blockIdx.x == 0) {
             if (threadIdx.x < w) y[(size_t)q * n + c0 + threadIdx.x] = yj[threadIdx.x];
         } else {
             const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
             double s = 0.0;
-            for (int c = qq; c < w; c += 4) s += Lt[r * TP + c] * yj[c];
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) {
+                const int c = qq + 4 * k;
+                s += Lt[r * TP + c] * yj[c];  // zero past the width on both sides
+            }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] -= s;
@@ -761,35 +797,43 @@ __global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const
     const double *Li = inv + (size_t)J * NB * NB;
     const int I = (int)blockIdx.x - 1;  // blockIdx 0 -> the diagonal block
     int c0 = 0, cn = 0;
+    Tile t{};
+    double lt[SE_T];
     if (blockIdx.x > 0) {
         c0 = P.start(I);
         cn = P.width(I);
-        const Tile t = tile_at(P, r0, c0);
-        for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-            int r, c;
-            if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-            Lt[r * TP + c] = (r < w && c < cn) ? (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs] : 0.0;
-        }
+        t = tile_at(P, r0, c0);
+        step_tile_load(a, t, w, cn, lt);
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
         {
             // x_J[c] = sum_r Linv[r][c] y_J[r]
             const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
+            const double *yq = y + (size_t)q * n + r0;
             double s = 0.0;
-            if (c < w)
-                for (int r = qq; r < w; r += 4) s += Li[r * NB + c] * y[(size_t)q * n + r0 + r];
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) {
+                const int r = qq + 4 * k;
+                const double lv = Li[r * NB + c], yv = yq[min(r, w - 1)];
+                s += r < w ? lv * yv : 0.0;
+            }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && c < NB) xj[c] = c < w ? s : 0.0;
         }
+        if (q == 0 && blockIdx.x > 0) step_tile_store(t, lt, Lt);
         __syncthreads();
         if (blockIdx.x == 0) {
             if (threadIdx.x < w) x[(size_t)q * n + r0 + threadIdx.x] = xj[threadIdx.x];
         } else {
             const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
             double s = 0.0;
-            for (int r = qq; r < w; r += 4) s += Lt[r * TP + c] * xj[r];
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) {
+                const int r = qq + 4 * k;
+                s += Lt[r * TP + c] * xj[r];  // zero past the width on both sides
+            }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && c < cn) y[(size_t)q * n + c0 + c] -= s;
