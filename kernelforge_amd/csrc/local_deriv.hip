@@ -404,9 +404,9 @@ __global__ __launch_bounds__(NT) void deriv_prep_kernel(DerivArgs A, int pair0) 
 //   V_i (3 n_x x ncol) = J_xi^T (3 n_x x rep) * D (rep x ncol),  D[f][c] = x_xi[f] - x_yj[f]
 // which streams each J_xi row once for the whole group (the one-pair form streamed it
 // once per pair).  The columns depend only on the label of atom i, so they are listed
-// once per label of x, and the (atom, 16-column fragment) items of a label are dealt
-// over the four waves; A operands (J rows) and B operands (differences, formed in
-// registers) come straight from HBM, 16 features ahead.
+// once per label of x, and the (atom, pair of 16-column fragments) items of a label are
+// dealt over the four waves; A operands (J rows, shared by the two fragments) and B
+// operands (differences, formed in registers) come straight from HBM, 16 features ahead.
 // ---------------------------------------------------------------------------
 constexpr int PG = 8;  // partners per group
 
@@ -508,27 +508,38 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
         }
         const int ncol = wbase[4];
         const int ncf = (ncol + 15) >> 4;
-        const int nitems = s_nl * ncf;
+        // items: an atom i of the label and a pair of column fragments (2 zp, 2 zp + 1); the
+        // two fragments share the streamed J_xi rows (eight MFMAs per k-step on one load of
+        // the A operands).  A second fragment past the last is computed on clamped columns
+        // and not stored.
+        const int npf2 = (ncf + 1) >> 1;
+        const int nitems = s_nl * npf2;
         for (int it = wv; it < nitems; it += 4) {
-        const int i = xl[it / ncf], z = it % ncf;
-        const double *xi = xx0 + (size_t)i * rep;
-        const double *Ji = Jx0 + (size_t)i * rep * lda;
-        {
-            const int c = min(16 * z + col, ncol - 1);
-            const int kc = cols[c] >> 5, jc = cols[c] & 31;
-            const double *xj = Xy + ((size_t)gy[kc] * may + jc) * rep;
-            v4d acc[NPF];
+            const int i = xl[it / npf2], zp = it % npf2;
+            const double *xi = xx0 + (size_t)i * rep;
+            const double *Ji = Jx0 + (size_t)i * rep * lda;
+            const double *xj[2];
 #pragma unroll
-            for (int y = 0; y < NPF; ++y) acc[y] = (v4d){0.0, 0.0, 0.0, 0.0};
-            double d2 = 0.0;
+            for (int h = 0; h < 2; ++h) {
+                const int c = min(16 * (2 * zp + h) + col, ncol - 1);
+                const int kc = cols[c] >> 5, jc = cols[c] & 31;
+                xj[h] = Xy + ((size_t)gy[kc] * may + jc) * rep;
+            }
+            v4d acc[2][NPF];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int y = 0; y < NPF; ++y) acc[h][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+            double d2[2] = {0.0, 0.0};
             // 16 features (4 k-steps) per block; the next block is loaded during this one
-            double ja[2][4][NPF], da[2][4], db[2][4];
+            double ja[2][4][NPF], da[2][4], db[2][2][4];
 #define KF_VP_LOAD(S, F0)                                                                      \
     do {                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                        \
             const int f_ = min((F0) + 4 * u + kr, rep - 1);                                    \
             da[S][u] = xi[f_];                                                                 \
-            db[S][u] = xj[f_];                                                                 \
+            db[S][0][u] = xj[0][f_];                                                           \
+            db[S][1][u] = xj[1][f_];                                                           \
             _Pragma("unroll") for (int y = 0; y < NPF; ++y)                                    \
                 ja[S][u][y] = Ji[(size_t)f_ * lda + pofs[y]];                                  \
         }                                                                                      \
@@ -536,10 +547,13 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
 #define KF_VP_MMA(S, F0)                                                                       \
     do {                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                        \
-            const double d_ = (F0) + 4 * u + kr < rep ? da[S][u] - db[S][u] : 0.0;             \
-            d2 = fma(d_, d_, d2);                                                              \
-            _Pragma("unroll") for (int y = 0; y < NPF; ++y)                                    \
-                acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ja[S][u][y], d_, acc[y], 0, 0, 0); \
+            const bool in_ = (F0) + 4 * u + kr < rep;                                          \
+            _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                    \
+                const double d_ = in_ ? da[S][u] - db[S][h][u] : 0.0;                          \
+                d2[h] = fma(d_, d_, d2[h]);                                                    \
+                _Pragma("unroll") for (int y = 0; y < NPF; ++y)                                \
+                    acc[h][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ja[S][u][y], d_, acc[h][y], 0, 0, 0); \
+            }                                                                                  \
         }                                                                                      \
     } while (0)
             KF_VP_LOAD(0, 0);
@@ -552,37 +566,40 @@ __global__ __launch_bounds__(NT) void deriv_vproj_kernel(DerivArgs A, const Proj
             }
 #undef KF_VP_LOAD
 #undef KF_VP_MMA
-            // |d|^2 of column c: the four lanes kr = 0..3 hold quarter sums
-            d2 += __shfl_xor(d2, 16, 64);
-            d2 += __shfl_xor(d2, 32, 64);
-            const int cc = 16 * z + col;
-            if (cc >= ncol) continue;
-            const int k = cols[cc] >> 5, j = cols[cc] & 31;
-            double *slot = A.slot + (size_t)gslot[k] * A.slot_len;
-            if (s0) {
-                const int m = ist[k][i] + rnk[k][j];
-                double *V = slot + A.maxm + (size_t)m * A.ldv;
-                if (kr == 0) slot[m] = exp(d2 * A.cexp);
-                // acc[y][v] is row p = 16y + 4v + kr of column cc
 #pragma unroll
-                for (int y = 0; y < NPF; ++y)
+            for (int h = 0; h < 2; ++h) {
+                // |d|^2 of column c: the four lanes kr = 0..3 hold quarter sums
+                double q2 = d2[h];
+                q2 += __shfl_xor(q2, 16, 64);
+                q2 += __shfl_xor(q2, 32, 64);
+                const int cc = 16 * (2 * zp + h) + col;
+                if (cc >= ncol) continue;
+                const int k = cols[cc] >> 5, j = cols[cc] & 31;
+                double *slot = A.slot + (size_t)gslot[k] * A.slot_len;
+                if (s0) {
+                    const int m = ist[k][i] + rnk[k][j];
+                    double *V = slot + A.maxm + (size_t)m * A.ldv;
+                    if (kr == 0) slot[m] = exp(q2 * A.cexp);
+                    // acc[h][y][v] is row p = 16y + 4v + kr of column cc
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int p = 16 * y + 4 * v + kr;
-                        if (p < ncx) V[p] = acc[y][v];
-                    }
-            } else {
-                const int m = ist[k][j] + rx[i];
-                double *U = slot + A.maxm + (size_t)A.maxm * A.ldv + (size_t)m * A.ldu;
+                    for (int y = 0; y < NPF; ++y)
 #pragma unroll
-                for (int y = 0; y < NPF; ++y)
+                        for (int v = 0; v < 4; ++v) {
+                            const int p = 16 * y + 4 * v + kr;
+                            if (p < ncx) V[p] = acc[h][y][v];
+                        }
+                } else {
+                    const int m = ist[k][j] + rx[i];
+                    double *U = slot + A.maxm + (size_t)A.maxm * A.ldv + (size_t)m * A.ldu;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int p = 16 * y + 4 * v + kr;
-                        if (p < ncx) U[p] = -acc[y][v];
-                    }
+                    for (int y = 0; y < NPF; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int p = 16 * y + 4 * v + kr;
+                            if (p < ncx) U[p] = -acc[h][y][v];
+                        }
+                }
             }
-        }
         }
     }
 }
