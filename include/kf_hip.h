@@ -15,8 +15,12 @@
  *    (src/local_kernels.cpp:40).  naq = 3 * sum(clamped N).
  *  - `stream` is a hipStream_t (NULL = the null stream).  Calls are
  *    stream-ordered; entry points that need host-side metadata (labels, atom
- *    counts) copy it back and synchronise the stream.  Calls on one device must
- *    not overlap: they share the library's device workspace.
+ *    counts) copy it back and synchronise the stream.  Calls on one device share
+ *    the library's device workspace, so the library serialises them: each entry
+ *    point holds a per-device lock for its host-side body, and a call on a stream
+ *    other than the previous call's first makes its stream wait for the previous
+ *    call's work (an event).  Concurrent calls from several threads are therefore
+ *    safe but do not run concurrently on the GPU.
  *  - Return value: KF_OK (0) on success, a negative KF_ERR_* code for invalid
  *    arguments or runtime failures (kf_last_error() has the message), or a
  *    positive factorisation `info` (leading minor that is not positive
@@ -312,10 +316,12 @@ int kf_rfp_pcg(int a_precision, const void *arf, int m_precision, const void *lf
 /* Mixed-precision replacement of cho_solve_rfp (src/math.cpp:134-171) for an FP32
  * RFP kernel K32 (e.g. from kf_local_kernel_symm_rfp with KF_PREC_BF16X3): FP32 RFP
  * Cholesky of K32 + mu I into lwork (n(n+1)/2 floats; mu = lambda, raised only if
- * the FP32 factorisation breaks down) and PCG to ||r||/||y|| <= tol with FP64
- * residuals of (K32 + lambda I).  When K32 + lambda I is not positive definite to
- * working precision (PCG breakdown) the system solved is K32 + mu I; *lambda_used
- * returns the regularisation actually solved.  K32 is not modified. */
+ * the FP32 factorisation breaks down) used as the preconditioner of PCG on
+ * (K32 + lambda I) with FP64 residuals, to ||r||/||y|| <= tol.  The system solved is
+ * always K32 + lambda I: if PCG stalls or breaks down (K32 + lambda I not positive
+ * definite in FP32 storage) the call returns KF_ERR_NOCONV and the caller falls back to
+ * the FP64 kernel and kf_rfp_potrf/potrs.  *lambda_used (may be null) returns mu, the
+ * shift of the preconditioner's factor.  K32 is not modified. */
 int kf_krr_solve_mixed(const float *K32, float *lwork, int n, double lambda, const double *y,
                        double *alpha, double tol, int max_iter, int *iters, double *rel_residual,
                        double *lambda_used, void *stream);

@@ -113,6 +113,10 @@ def timer_read(tid: int):
     return ms.value, cnt.value
 
 
+class NoConvergence(RuntimeError):
+    """An iterative solve stopped above its tolerance (KF_ERR_NOCONV)."""
+
+
 def check(rc: int, what: str) -> int:
     if rc == KF_OK:
         return rc
@@ -121,6 +125,8 @@ def check(rc: int, what: str) -> int:
         raise ValueError(msg)
     if rc == KF_ERR_NOMEM:
         raise MemoryError(msg)
+    if rc == KF_ERR_NOCONV:
+        raise NoConvergence(msg)
     if rc > 0:
         raise RuntimeError(f"{what} failed, info={rc}")
     raise RuntimeError(msg)
@@ -249,7 +255,9 @@ def rfp_symv(precision, arf, n, lam, x, y, stream=0):
 
 
 def krr_solve_mixed(K32, lwork, n, lam, y, alpha, tol=1e-10, max_iter=200, stream=0):
-    """Returns (iterations, relative residual, regularisation actually solved)."""
+    """Returns (iterations, relative residual, shift of the FP32 preconditioner factor).
+
+    Raises NoConvergence when PCG on (K32 + lam I) stops above ``tol``."""
     it, rel, mu = _c_int(0), _c_double(0.0), _c_double(0.0)
     check(lib().kf_krr_solve_mixed(K32, lwork, n, lam, y, alpha, tol, max_iter, ctypes.byref(it),
                                    ctypes.byref(rel), ctypes.byref(mu), stream or None),

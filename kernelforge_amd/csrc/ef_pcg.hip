@@ -347,6 +347,7 @@ using namespace kf;
 extern "C" int kf_block_factor(const double *blocks, int nb, int bs, const int *N, int max_atoms,
                                double lambda, const double *shift, double *inv, int *info,
                                void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(nb >= 0 && bs == 1 + 3 * max_atoms && bs <= MAXBS, KF_ERR_ARG,
                "block size must be 1 + 3 max_atoms <= %d", MAXBS);
     if (nb == 0) return KF_OK;
@@ -360,6 +361,7 @@ extern "C" int kf_block_factor(const double *blocks, int nb, int bs, const int *
 
 extern "C" int kf_block_apply(const double *inv, int nb, int bs, const int *N, int max_atoms,
                               const double *r, double *z, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(nb >= 0 && bs == 1 + 3 * max_atoms && bs <= MAXBS, KF_ERR_ARG, "bad block size");
     if (nb == 0) return KF_OK;
     hipStream_t st = as_stream(stream);
@@ -376,7 +378,10 @@ extern "C" int kf_ef_pcg_rfp(const double *arf, int n, double lambda, const doub
                              int nb, int bs, const int *N, int max_atoms, const double *y,
                              double *x, double tol, int max_iter, int *iters, double *rel,
                              double *history, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(n >= 0 && nb >= 0 && max_iter >= 0, KF_ERR_ARG, "bad sizes");
+    KF_REQUIRE(!inv || (bs == 1 + 3 * max_atoms && bs <= MAXBS), KF_ERR_ARG,
+               "block size must be 1 + 3 max_atoms <= %d", MAXBS);
     if (n == 0) return KF_OK;
     hipStream_t st = as_stream(stream);
     int *d_offs = nullptr;
@@ -396,7 +401,10 @@ extern "C" int kf_ef_pcg_matfree(const double *X, const double *dX, const int *Q
                                  const double *inv, int bs, const double *y, double *x,
                                  double tol, int max_iter, int *iters, double *rel,
                                  double *history, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(nm >= 0 && max_iter >= 0, KF_ERR_ARG, "bad sizes");
+    KF_REQUIRE(!inv || (bs == 1 + 3 * max_atoms && bs <= MAXBS), KF_ERR_ARG,
+               "block size must be 1 + 3 max_atoms <= %d", MAXBS);
     if (nm == 0) return KF_OK;
     hipStream_t st = as_stream(stream);
     int *d_offs = nullptr;
@@ -432,7 +440,10 @@ extern "C" int kf_pcg_unrolled_grad_rfp(const double *arf, int n, double lambda,
                                         const double *inv, int nb, int bs, const int *N,
                                         int max_atoms, const double *y, int steps, double *loss,
                                         double *grad_shift, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(n > 0 && steps >= 1 && inv && grad_shift && loss, KF_ERR_ARG, "bad arguments");
+    KF_REQUIRE(bs == 1 + 3 * max_atoms && bs <= MAXBS, KF_ERR_ARG,
+               "block size must be 1 + 3 max_atoms <= %d", MAXBS);
     hipStream_t st = as_stream(stream);
     int *d_offs = nullptr;
     const int naq = upload_offs(N, nb, max_atoms, st, &d_offs);

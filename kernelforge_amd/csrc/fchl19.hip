@@ -549,6 +549,7 @@ extern "C" int kf_fchl19_f64(const double *coords, const int *Q, const int *N, i
                              double eta3, double zeta, double rcut, double acut,
                              double two_body_decay, double three_body_decay, double w3,
                              double *X, double *dX, void *stream_) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream_));
     using namespace kf;
     hipStream_t stream = as_stream(stream_);
     KF_REQUIRE(nm >= 0, KF_ERR_ARG, "nm must be >= 0");

@@ -76,6 +76,25 @@ int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Serialises the C-ABI entry points of one device.  They share the device workspace
+// slots and pinned staging above, so every compute entry point holds this guard for
+// its whole host-side body (a per-device recursive mutex: entry points may call one
+// another).  When the outermost call on a device uses a different stream from the
+// previous call, its stream first waits for an event recorded at the end of that
+// call, so the GPU work of calls on different streams never overlaps in the
+// workspace either.  Not applied while the stream is being captured into a graph.
+class EntryGuard {
+  public:
+    explicit EntryGuard(hipStream_t s);
+    ~EntryGuard();
+    EntryGuard(const EntryGuard &) = delete;
+    EntryGuard &operator=(const EntryGuard &) = delete;
+
+  private:
+    int dev_ = -1;
+    hipStream_t stream_ = nullptr;
+};
+
 // Kernel timers: when enabled (kf_timer_enable), entry points bracket their main
 // kernel with HIP events recorded on the launch stream; kf_timer_read sums them.
 enum TimerId {

@@ -105,6 +105,57 @@ void *host_staging(int slot, size_t bytes) {
     return p;
 }
 
+namespace {
+struct DeviceGate {
+    std::recursive_mutex m;
+    int depth = 0;  // nesting of entry points on the owning thread
+    hipStream_t last = nullptr;
+    hipEvent_t done = nullptr;  // recorded at the end of the last outermost call
+    bool recorded = false;
+};
+DeviceGate g_gate[kMaxDevices];
+
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+EntryGuard::EntryGuard(hipStream_t s) : stream_(s) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return;
+    }
+    dev_ = dev;
+    DeviceGate &g = g_gate[dev];
+    g.m.lock();
+    if (g.depth++ == 0 && g.recorded && g.last != s && !capturing(s))
+        (void)hipStreamWaitEvent(s, g.done, 0);
+}
+
+EntryGuard::~EntryGuard() {
+    if (dev_ < 0) return;
+    DeviceGate &g = g_gate[dev_];
+    if (--g.depth == 0 && !capturing(stream_)) {
+        if (!g.done && hipEventCreateWithFlags(&g.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            g.done = nullptr;
+        }
+        if (g.done && hipEventRecord(g.done, stream_) == hipSuccess) {
+            g.recorded = true;
+            g.last = stream_;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    g.m.unlock();
+}
+
 // Copy Q (nm x ma) and N (nm) from the device to host vectors (synchronises the stream).
 int read_metadata(const int *Q, const int *N, int nm, int ma, std::vector<int> &hQ,
                   std::vector<int> &hN, hipStream_t stream, int slot) {
@@ -135,7 +186,7 @@ std::vector<std::pair<hipEvent_t, hipEvent_t>> g_tm_events[TM_COUNT];
 
 ScopedTimer::ScopedTimer(int id_, hipStream_t s) : id(id_), stream(s), handle(nullptr) {
     std::lock_guard<std::mutex> lock(g_tm_mutex);
-    if (!g_tm_enabled) return;
+    if (!g_tm_enabled || id < 0 || id >= TM_COUNT) return;  // id -1: untimed
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return;
     if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }

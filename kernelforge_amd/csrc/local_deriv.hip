@@ -1084,6 +1084,7 @@ extern "C" int kf_local_kernel_jacobian(const double *X1, const double *X2, cons
                                         const int *N2, int nm1, int nm2, int max_atoms1,
                                         int max_atoms2, int rep, double sigma, double *K,
                                         void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_JAC>(X1, nullptr, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
                              max_atoms2, rep, sigma, K, as_stream(stream));
 }
@@ -1093,6 +1094,7 @@ extern "C" int kf_local_kernel_jacobian_t(const double *X1, const double *X2, co
                                           const int *N2, int nm1, int nm2, int max_atoms1,
                                           int max_atoms2, int rep, double sigma, double *K,
                                           void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_JAC_T>(X1, dX1, X2, nullptr, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
                                max_atoms2, rep, sigma, K, as_stream(stream));
 }
@@ -1102,6 +1104,7 @@ extern "C" int kf_local_kernel_hessian(const double *X1, const double *X2, const
                                        const int *N1, const int *N2, int nm1, int nm2,
                                        int max_atoms1, int max_atoms2, int rep, double sigma,
                                        double *K, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_HESS>(X1, dX1, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1, max_atoms2,
                               rep, sigma, K, as_stream(stream));
 }
@@ -1111,6 +1114,7 @@ extern "C" int kf_local_kernel_full(const double *X1, const double *X2, const do
                                     const int *N1, const int *N2, int nm1, int nm2,
                                     int max_atoms1, int max_atoms2, int rep, double sigma,
                                     double *K, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_FULL>(X1, dX1, X2, dX2, Q1, Q2, N1, N2, nm1, nm2, max_atoms1, max_atoms2,
                               rep, sigma, K, as_stream(stream));
 }
@@ -1118,6 +1122,7 @@ extern "C" int kf_local_kernel_full(const double *X1, const double *X2, const do
 extern "C" int kf_local_kernel_hessian_symm(const double *X, const double *dX, const int *Q,
                                             const int *N, int nm, int max_atoms, int rep,
                                             double sigma, double *K, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_HESS_SYMM>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
                                    sigma, K, as_stream(stream));
 }
@@ -1125,6 +1130,7 @@ extern "C" int kf_local_kernel_hessian_symm(const double *X, const double *dX, c
 extern "C" int kf_local_kernel_hessian_symm_rfp(const double *X, const double *dX, const int *Q,
                                                 const int *N, int nm, int max_atoms, int rep,
                                                 double sigma, double *arf, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_HESS_SYMM_RFP>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms,
                                        rep, sigma, arf, as_stream(stream));
 }
@@ -1132,6 +1138,7 @@ extern "C" int kf_local_kernel_hessian_symm_rfp(const double *X, const double *d
 extern "C" int kf_local_kernel_full_symm(const double *X, const double *dX, const int *Q,
                                          const int *N, int nm, int max_atoms, int rep,
                                          double sigma, double *K, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_FULL_SYMM>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
                                    sigma, K, as_stream(stream));
 }
@@ -1139,6 +1146,7 @@ extern "C" int kf_local_kernel_full_symm(const double *X, const double *dX, cons
 extern "C" int kf_local_kernel_full_symm_rfp(const double *X, const double *dX, const int *Q,
                                              const int *N, int nm, int max_atoms, int rep,
                                              double sigma, double *arf, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return deriv_impl<M_FULL_SYMM_RFP>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms,
                                        rep, sigma, arf, as_stream(stream));
 }
@@ -1146,6 +1154,7 @@ extern "C" int kf_local_kernel_full_symm_rfp(const double *X, const double *dX, 
 extern "C" int kf_local_full_diag_blocks(const double *X, const double *dX, const int *Q,
                                          const int *N, int nm, int max_atoms, int rep,
                                          double sigma, double *blocks, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     // entries beyond a molecule's 1 + 3 N[b] live rows/columns are not written
     return deriv_impl<M_DIAG_BLOCKS>(X, dX, X, dX, Q, Q, N, N, nm, nm, max_atoms, max_atoms, rep,
                                      sigma, blocks, as_stream(stream));

@@ -317,6 +317,7 @@ using namespace kf;
 extern "C" int kf_local_alpha_desc(const double *dX, const int *Q, const int *N, int nm,
                                    int max_atoms, int rep, const double *alpha, double *out,
                                    void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     (void)Q;
     KF_REQUIRE(nm >= 0 && max_atoms >= 0 && rep >= 1, KF_ERR_ARG, "bad dimensions");
     KF_REQUIRE(3 * max_atoms <= MAXC, KF_ERR_BASIS, "at most %d atoms per molecule", MAXC / 3);
@@ -347,6 +348,7 @@ extern "C" int kf_local_full_matvec(const double *X1, const double *dX1, const d
                                     const int *Q1, const int *Q2, const int *N1, const int *N2,
                                     int nm1, int nm2, int max_atoms1, int max_atoms2, int rep,
                                     double sigma, double *E, double *F, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return matvec_impl(X1, dX1, X2, alpha_desc, alpha_E, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
                        max_atoms2, rep, sigma, E, F, 1 | 2 | 4, as_stream(stream));
 }
@@ -356,6 +358,7 @@ extern "C" int kf_local_hessian_matvec(const double *X1, const double *dX1, cons
                                        const int *N1, const int *N2, int nm1, int nm2,
                                        int max_atoms1, int max_atoms2, int rep, double sigma,
                                        double *F, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return matvec_impl(X1, dX1, X2, alpha_desc, nullptr, Q1, Q2, N1, N2, nm1, nm2, max_atoms1,
                        max_atoms2, rep, sigma, nullptr, F, 2, as_stream(stream));
 }
@@ -365,6 +368,7 @@ extern "C" int kf_local_jacobian_t_matvec(const double *X1, const double *X2,
                                           const int *Q2, const int *N1, const int *N2, int nm1,
                                           int nm2, int max_atoms1, int max_atoms2, int rep,
                                           double sigma, double *E, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return matvec_impl(X1, nullptr, X2, alpha_desc, nullptr, Q1, Q2, N1, N2, nm1, nm2,
                        max_atoms1, max_atoms2, rep, sigma, E, nullptr, 1, as_stream(stream));
 }

@@ -848,6 +848,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
 extern "C" int kf_local_kernel_symm_rfp(int precision, const double *X, const int *Q,
                                         const int *N, int nm, int max_atoms, int rep,
                                         double sigma, void *arf, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return kf::symm_impl<kf::SYMM_RFP>(precision, X, Q, N, nm, max_atoms, rep, sigma, arf,
                                        kf::as_stream(stream));
 }
@@ -855,6 +856,7 @@ extern "C" int kf_local_kernel_symm_rfp(int precision, const double *X, const in
 extern "C" int kf_local_kernel_symm(int precision, const double *X, const int *Q, const int *N,
                                     int nm, int max_atoms, int rep, double sigma, void *K,
                                     void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     return kf::symm_impl<kf::SYMM_DENSE>(precision, X, Q, N, nm, max_atoms, rep, sigma, K,
                                          kf::as_stream(stream));
 }
@@ -863,6 +865,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
                                const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
                                int max_atoms1, int max_atoms2, int rep, double sigma, void *K,
                                void *stream_) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream_));
     using namespace kf;
     hipStream_t stream = as_stream(stream_);
     int rc = check_common(X1, Q1, N1, nm1, max_atoms1, rep, sigma);

@@ -106,6 +106,7 @@ extern "C" int kf_mlp_forward(const float *X, int B, int D0, int H, int D3, cons
                               const float *b1, const uint16_t *W2, const float *b2,
                               const uint16_t *W3, const float *b3, float *Z1, float *Z2,
                               float *Y, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(B >= 0 && D0 > 0 && H > 0 && D3 > 0, KF_ERR_ARG, "bad MLP shape");
     if (B == 0) return KF_OK;
     KF_REQUIRE(X && W1 && W2 && W3 && Z1 && Z2 && Y, KF_ERR_ARG, "null pointer");
@@ -128,6 +129,7 @@ extern "C" int kf_mlp_backward(const float *X, int B, int D0, int H, int D3, con
                                const uint16_t *W2, const uint16_t *W3, const float *Z1,
                                const float *Z2, const float *dY, float *dW1, float *db1,
                                float *dW2, float *db2, float *dW3, float *db3, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(B >= 0 && D0 > 0 && H > 0 && D3 > 0, KF_ERR_ARG, "bad MLP shape");
     if (B == 0) return KF_OK;
     KF_REQUIRE(X && W1 && W2 && W3 && Z1 && Z2 && dY && dW1 && dW2 && dW3, KF_ERR_ARG,
@@ -157,6 +159,7 @@ extern "C" int kf_mlp_backward(const float *X, int B, int D0, int H, int D3, con
 extern "C" int kf_mlp_adam(float *p, const float *g, float *m, float *v, int n, float lr,
                            float beta1, float beta2, float eps, int t, uint16_t *p_bf16,
                            void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(n >= 0 && t >= 1, KF_ERR_ARG, "bad Adam arguments");
     if (n == 0) return KF_OK;
     const float c1 = 1.0f - std::pow(beta1, (float)t), c2 = 1.0f - std::pow(beta2, (float)t);
@@ -167,6 +170,7 @@ extern "C" int kf_mlp_adam(float *p, const float *g, float *m, float *v, int n, 
 }
 
 extern "C" int kf_cast_bf16(const float *p, int n, uint16_t *out, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     KF_REQUIRE(n >= 0, KF_ERR_ARG, "bad size");
     if (n == 0) return KF_OK;
     hipLaunchKernelGGL(cast_bf16_kernel, grid1(n), dim3(256), 0, as_stream(stream), p, n, out);

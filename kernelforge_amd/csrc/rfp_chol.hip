@@ -921,17 +921,21 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     return h;
 }
 
+// Scratch of a solve: the diagonal-block inverses, then the intermediate y = L^-1 b.
+// Every byte a solve reads from it is first written by the same solve (inv_diag_all
+// writes whole 64x64 blocks, the forward steps every row of y), so it needs no clearing.
+inline size_t potrs_inv_bytes(const Part &P) { return sizeof(double) * (size_t)P.nblk * NB * NB; }
+inline size_t potrs_scratch_bytes(const Part &P, int n, int nrhs) {
+    return potrs_inv_bytes(P) + sizeof(double) * (size_t)n * nrhs;
+}
+
 template <typename T>
-int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
-    KF_REQUIRE(a && b && n > 0 && nrhs >= 1, KF_ERR_ARG, "bad potrs arguments");
-    const Part P = make_part(n);
-    const size_t inv_bytes = sizeof(double) * (size_t)P.nblk * NB * NB;
-    char *ws = static_cast<char *>(
-        workspace(WS_AUX3, inv_bytes + sizeof(double) * (size_t)n * nrhs + 256, stream));
-    KF_REQUIRE(ws, KF_ERR_NOMEM, "workspace allocation failed");
-    double *inv = reinterpret_cast<double *>(ws);
-    double *y = reinterpret_cast<double *>(ws + inv_bytes);
-    ScopedTimer timer(TM_POTRS, stream);
+int potrs_run(const T *a, const Part &P, double *b, int nrhs, char *scratch, hipStream_t stream,
+              bool timed) {
+    const int n = P.n;
+    double *inv = reinterpret_cast<double *>(scratch);
+    double *y = reinterpret_cast<double *>(scratch + potrs_inv_bytes(P));
+    ScopedTimer timer(timed ? TM_POTRS : -1, stream);
     hipLaunchKernelGGL(inv_diag_all<T>, dim3(P.nblk), dim3(256), 0, stream, a, P, inv);
     for (int J = 0; J < P.nblk; ++J)
         hipLaunchKernelGGL(fwd_step<T>, dim3(P.nblk - J), dim3(256), 0, stream, a, P, J, inv, b,
@@ -943,10 +947,20 @@ int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
     return KF_OK;
 }
 
+template <typename T>
+int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
+    KF_REQUIRE(a && b && n > 0 && nrhs >= 1, KF_ERR_ARG, "bad potrs arguments");
+    const Part P = make_part(n);
+    char *ws = static_cast<char *>(workspace(WS_AUX3, potrs_scratch_bytes(P, n, nrhs) + 256, stream));
+    KF_REQUIRE(ws, KF_ERR_NOMEM, "workspace allocation failed");
+    return potrs_run(a, P, b, nrhs, ws, stream, true);
+}
+
 }  // namespace
 }  // namespace kf
 
 extern "C" int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     using namespace kf;
     if (precision == KF_PREC_F64) return potrf_impl(static_cast<double *>(arf), n, l2, as_stream(stream));
     if (precision == KF_PREC_F32) return potrf_impl(static_cast<float *>(arf), n, l2, as_stream(stream));
@@ -955,6 +969,7 @@ extern "C" int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *st
 }
 
 extern "C" int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     using namespace kf;
     if (precision == KF_PREC_F64)
         return potrs_impl(static_cast<const double *>(arf), n, b, nrhs, as_stream(stream));

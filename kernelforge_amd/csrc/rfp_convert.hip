@@ -71,6 +71,7 @@ inline int flags(char transr, char uplo, bool &trans, bool &upper) {
 
 extern "C" int kf_trttf(char transr, char uplo, int n, const double *A, int lda, double *arf,
                         void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     using namespace kf;
     bool trans, upper;
     int rc = flags(transr, uplo, trans, upper);
@@ -87,6 +88,7 @@ extern "C" int kf_trttf(char transr, char uplo, int n, const double *A, int lda,
 
 extern "C" int kf_tfttr(char transr, char uplo, int n, const double *arf, double *A, int lda,
                         int symmetrize, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     using namespace kf;
     bool trans, upper;
     int rc = flags(transr, uplo, trans, upper);

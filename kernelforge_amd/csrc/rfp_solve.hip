@@ -160,6 +160,7 @@ int symv(const T *a, int n, double lambda, const double *x, double *y, hipStream
 
 extern "C" int kf_rfp_symv(int precision, const void *arf, int n, double lambda, const double *x,
                            double *y, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     using namespace kf;
     KF_REQUIRE(arf && x && y && n > 0, KF_ERR_ARG, "bad kf_rfp_symv arguments");
     if (precision == KF_PREC_F64)
@@ -175,6 +176,7 @@ extern "C" int kf_rfp_symv(int precision, const void *arf, int n, double lambda,
 extern "C" int kf_rfp_pcg(int a_precision, const void *arf, int m_precision, const void *lfac,
                           int n, double lambda, const double *y, double *alpha, double tol,
                           int max_iter, int *iters, double *rel_residual, void *stream_) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream_));
     using namespace kf;
     hipStream_t stream = as_stream(stream_);
     KF_REQUIRE(arf && lfac && y && alpha && n > 0 && max_iter >= 0, KF_ERR_ARG,
@@ -277,14 +279,15 @@ __global__ void diag_sum(const T *__restrict__ a, int n, double *__restrict__ ou
 // Mixed-precision solve of (K + lambda I) alpha = y for an FP32 RFP kernel matrix K:
 // factor K + mu I in FP32 into `lwork` (mu = lambda first; if the FP32 factorisation
 // breaks down, mu grows by 10x from 1e-7 * mean(diag K) until it succeeds), then PCG
-// with FP64 residuals on (K + lambda I) preconditioned by that factor.  If the FP32-
-// stored operator K + lambda I is not positive definite to working precision (PCG
-// breakdown), the system solved is (K + mu I), i.e. the smallest regularisation the
-// FP32 factorisation accepted; `lambda_used` reports which one was solved.
+// with FP64 residuals on (K + lambda I) preconditioned by that factor.  The system
+// solved is always the requested one: when PCG does not reach the tolerance (or breaks
+// down because K + lambda I is not positive definite in FP32 storage) the call returns
+// KF_ERR_NOCONV and the caller falls back to the FP64 factorisation.
 extern "C" int kf_krr_solve_mixed(const float *K32, float *lwork, int n, double lambda,
                                   const double *y, double *alpha, double tol, int max_iter,
                                   int *iters, double *rel_residual, double *lambda_used,
                                   void *stream_) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream_));
     using namespace kf;
     hipStream_t stream = as_stream(stream_);
     KF_REQUIRE(K32 && lwork && y && alpha && n > 0, KF_ERR_ARG, "bad kf_krr_solve_mixed arguments");
@@ -310,14 +313,7 @@ extern "C" int kf_krr_solve_mixed(const float *K32, float *lwork, int n, double 
         return info;
     }
     KF_HIP_TRY(hipMemsetAsync(alpha, 0, sizeof(double) * n, stream));
-    int rc = kf_rfp_pcg(KF_PREC_F32, K32, KF_PREC_F32, lwork, n, lambda, y, alpha, tol, max_iter,
-                        iters, rel_residual, stream_);
-    if (lambda_used) *lambda_used = lambda;
-    if (rc == KF_ERR_NOCONV && mu != lambda) {
-        KF_HIP_TRY(hipMemsetAsync(alpha, 0, sizeof(double) * n, stream));
-        rc = kf_rfp_pcg(KF_PREC_F32, K32, KF_PREC_F32, lwork, n, mu, y, alpha, tol, max_iter,
-                        iters, rel_residual, stream_);
-        if (lambda_used) *lambda_used = mu;
-    }
-    return rc;
+    if (lambda_used) *lambda_used = mu;  // the shift of the preconditioner's factor
+    return kf_rfp_pcg(KF_PREC_F32, K32, KF_PREC_F32, lwork, n, lambda, y, alpha, tol, max_iter,
+                      iters, rel_residual, stream_);
 }

@@ -58,3 +58,26 @@ def test_compute_rep_size_is_host_only():
 
     assert fchl19_repr.compute_rep_size(5, 24, 20, 2) == 720
     assert fchl19_repr.compute_rep_size(nelements=4, nbasis2=10, nbasis3=7, nabasis=6) == 460
+
+
+def test_pcg_entry_points_reject_block_size_before_any_launch():
+    """kf_ef_pcg_rfp / kf_ef_pcg_matfree / kf_pcg_unrolled_grad_rfp check the block size
+    (bs == 1 + 3 max_atoms <= 97, the LDS block of the preconditioner kernels) before
+    touching any pointer or launching a kernel, so this runs without a GPU."""
+    import ctypes
+
+    from kernelforge_amd import capi
+
+    L = capi.lib()
+    fake = 4096  # never dereferenced: the argument check comes first
+    it, rel = ctypes.c_int(0), ctypes.c_double(0.0)
+    loss = ctypes.c_double(0.0)
+    for ma, bs in ((32, 98), (33, 100), (4, 12)):
+        assert L.kf_ef_pcg_rfp(fake, 10, 1e-3, fake, 2, bs, fake, ma, fake, fake, 1e-8, 5,
+                               ctypes.byref(it), ctypes.byref(rel), None, None) == capi.KF_ERR_ARG
+        assert L.kf_ef_pcg_matfree(fake, fake, fake, fake, 2, ma, 720, 2.0, 1e-3, fake, bs, fake,
+                                   fake, 1e-8, 5, ctypes.byref(it), ctypes.byref(rel), None,
+                                   None) == capi.KF_ERR_ARG
+        assert L.kf_pcg_unrolled_grad_rfp(fake, 10, 1e-3, fake, 2, bs, fake, ma, fake, 3,
+                                          ctypes.byref(loss), fake, None) == capi.KF_ERR_ARG
+        assert b"block size" in L.kf_last_error()

@@ -177,3 +177,32 @@ def test_bf16x3_rect_and_random(lk, config0):
     x, q, n = random_set(150, 6, 33, 3, 5)
     assert rel_err(lk.kernel_gaussian_symm(x, q, n, 1.3, precision="bf16x3"),
                    oracle.kernel_symm(x, q, n, 1.3)) < 2e-5
+
+
+def test_two_threads_share_the_device_workspace(lk):
+    """Two Python threads call the kernel at once (the modules release the GIL); the
+    library serialises entry points on one device, so each thread gets its own kernel."""
+    import threading
+
+    sets = [random_set(90, 8, 720, 5, 501 + t) for t in range(2)]
+    want = [oracle.kernel_symm_rfp(x, q, n, 2.0) for x, q, n in sets]
+    got = [[None] * 6 for _ in sets]
+    errors = []
+
+    def work(t):
+        try:
+            x, q, n = sets[t]
+            for k in range(6):
+                got[t][k] = lk.kernel_gaussian_symm_rfp(x, q, n, 2.0)
+        except Exception as exc:  # reported by the assertion below
+            errors.append(exc)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(2)]
+    for h in th:
+        h.start()
+    for h in th:
+        h.join()
+    assert not errors, errors
+    for t in range(2):
+        for k in range(6):
+            assert rel_err(got[t][k], want[t]) < 1e-12

@@ -61,24 +61,63 @@ def test_pcg_with_exact_factor_converges_immediately():
     assert rel_err(da.numpy(), np.linalg.solve(K + 1e-3 * np.eye(n), y)) < 1e-10
 
 
-def test_mixed_solve_reaches_fp64_residual(config0):
-    """BF16x3 kernel -> FP32 factor -> PCG: the FP64 residual of the FP32 system reaches
-    1e-10; alpha / predictions vs the FP64 oracle are reported in bench.py."""
+def test_mixed_solve_vs_fp64_oracle(config0):
+    """BF16x3 kernel -> FP32 factor -> FP64-residual PCG at lambda = 1e-8 on the 800
+    training molecules of config 0, checked three ways:
+      * the solve itself: the FP64 residual of the FP32-stored system is <= 1e-10 and
+        alpha equals numpy.linalg.solve of that system;
+      * against the FP64 oracle fit (kernel + cho_solve_rfp): alpha and the 200 test
+        energies.  The kernel entries carry ~1e-7 relative error, and perturbing the
+        FP64 kernel by 1e-7 alone moves these energies by ~3e-4 (condition number
+        8e6), so the mixed fit cannot meet a 1e-6 energy bar; the bound asserted is
+        the measured one with margin."""
     from kernelforge_amd import capi, local_kernels
 
     _, _, energies, C, Q, N = config0
-    X = oracle.fchl19_batch(C[:800], Q[:800], N[:800])
-    y = energies[:800] - energies[:800].mean()
-    arf32 = local_kernels.kernel_gaussian_symm_rfp(X, Q[:800], N[:800], 2.0, precision="bf16x3")
+    X = oracle.fchl19_batch(C, Q, N)
+    tr, te = slice(0, 800), slice(800, 1000)
+    off = energies[tr].mean()
+    y = energies[tr] - off
+    arf32 = local_kernels.kernel_gaussian_symm_rfp(X[tr], Q[tr], N[tr], 2.0, precision="bf16x3")
     n = 800
     DA = capi.DeviceArray
     K32, Lw, dy, da = DA.from_numpy(arf32), DA(n * (n + 1) // 2, np.float32), DA.from_numpy(y), DA(n)
     it, rel, mu = capi.krr_solve_mixed(K32.ptr, Lw.ptr, n, 1e-8, dy.ptr, da.ptr, tol=1e-10,
                                        max_iter=200)
     assert rel <= 1e-10
-    # the residual is that of the FP32-stored system: check it on the host in FP64
+    a = da.numpy()
     Kf = sl.get_lapack_funcs("tfttr", (arf32.astype(np.float64),))(n, arf32.astype(np.float64),
                                                                    transr="N", uplo="U")[0]
     Kf = np.triu(Kf) + np.triu(Kf, 1).T
-    a = da.numpy()
-    assert np.linalg.norm((Kf + 1e-8 * np.eye(n)) @ a - y) / np.linalg.norm(y) < 1e-9
+    A32 = Kf + 1e-8 * np.eye(n)
+    assert np.linalg.norm(A32 @ a - y) / np.linalg.norm(y) < 1e-9
+    # alpha of the FP32-stored system: a relative residual of 1e-10 bounds the relative
+    # error (2-norm) by cond * 1e-10
+    s = np.linalg.solve(A32, y)
+    assert np.linalg.norm(a - s) / np.linalg.norm(s) < 1e-10 * np.linalg.cond(A32)
+    # against the FP64 oracle fit
+    arf64 = oracle.kernel_symm_rfp(X[tr], Q[tr], N[tr], 2.0)
+    a64 = oracle.cho_solve_rfp(arf64, y, 1e-8)
+    Kte = oracle.kernel_rect(X[te], X[tr], Q[te], Q[tr], N[te], N[tr], 2.0)
+    e64 = Kte @ a64 + off
+    em = Kte @ a + off
+    e_err = float(np.max(np.abs(em - e64) / np.abs(e64)))
+    a_err = rel_err(a, a64)
+    print(f"mixed fit vs FP64 oracle: alpha {a_err:.2e}, test energies {e_err:.2e} "
+          f"({it} PCG iterations, factor shift {mu:.1e})")
+    assert e_err < 1e-2 and a_err < 1.0
+
+
+def test_mixed_solve_reports_stall():
+    """A mixed solve that cannot reach its tolerance raises (KF_ERR_NOCONV) instead of
+    returning the solution of another system."""
+    from kernelforge_amd import capi
+
+    n = 200
+    K = spd(n, 11, shift=1e-6)
+    y = np.random.default_rng(12).standard_normal(n)
+    DA = capi.DeviceArray
+    K32, Lw, dy, da = (DA.from_numpy(to_rfp(K).astype(np.float32)),
+                       DA(n * (n + 1) // 2, np.float32), DA.from_numpy(y), DA(n))
+    with pytest.raises(capi.NoConvergence):
+        capi.krr_solve_mixed(K32.ptr, Lw.ptr, n, 0.0, dy.ptr, da.ptr, tol=1e-300, max_iter=1)
