@@ -215,6 +215,27 @@ int kf_local_jacobian_t_matvec(const double *X1, const double *X2, const double 
                                int nm1, int nm2, int max_atoms1, int max_atoms2, int rep,
                                double sigma, double *E, void *stream);
 
+/* Planned full matvec for repeated products with a fixed training side and fixed query
+ * labels (molecular dynamics: the query geometry changes, its atoms and labels do not).
+ * kf_matvec_plan_create reads the labels once, packs the training rows of every label
+ * with their norms, alpha_desc dot products and alpha_E (the quantities of the
+ * reference's precompute_train, src/cuda_local_kernels.hpp:178-230) and allocates the
+ * plan's own device workspace; kf_matvec_plan_execute then computes
+ * (E (nm1,), F (naq1,)) = kf_local_full_matvec(X1, dX1, ...) for new query descriptors
+ * and Jacobians with no host synchronisation and no allocation, so it can be captured
+ * into a graph once the plan exists.  Replaces kernel_gaussian_full_matvec_cached and
+ * _cached_graph (src/cuda_local_kernels.hpp:232-289).  A plan belongs to the device that
+ * was current at creation; executions on one plan must not overlap (they share its
+ * workspace).  kf_matvec_plan_destroy waits for the device before freeing. */
+typedef struct kf_matvec_plan kf_matvec_plan;
+int kf_matvec_plan_create(const int *Q1, const int *N1, int nm1, int max_atoms1, const double *X2,
+                          const double *alpha_desc, const double *alpha_E, const int *Q2,
+                          const int *N2, int nm2, int max_atoms2, int rep, double sigma,
+                          kf_matvec_plan **plan, void *stream);
+int kf_matvec_plan_execute(const kf_matvec_plan *plan, const double *X1, const double *dX1,
+                           double *E, double *F, void *stream);
+int kf_matvec_plan_destroy(kf_matvec_plan *plan);
+
 /* ------------------------------------------------------------------------ */
 /* Energy+force solve by PCG with a (learned) block-Jacobi preconditioner.    */
 /* The E+F vector order is [E_0..E_{nm-1}, F_0 (3 N_0), F_1, ...]; block b    */
@@ -336,6 +357,14 @@ int kf_trttf(char transr, char uplo, int n, const double *A, int lda, double *ar
  * other triangle is mirrored as kf::math::rfp_to_full does (src/math.cpp:181-211). */
 int kf_tfttr(char transr, char uplo, int n, const double *arf, double *A, int lda, int symmetrize,
              void *stream);
+
+/* Symmetric RFP re-layout between the four TRANSR x UPLO variants (column-major LAPACK
+ * semantics), FP64 in, FP64 or FP32 (out_precision) out.  No reference counterpart as a
+ * function: it converts this library's native packing (TRANSR='N', UPLO='U', that of
+ * src/local_kernels.cpp:386-400) to the reference GPU kernels' TRANSR='N', UPLO='L'
+ * (src/cuda_local_kernels.hpp:52) for the torch-tensor interface. */
+int kf_rfp_relayout(char transr_in, char uplo_in, char transr_out, char uplo_out, int n,
+                    const double *in, int out_precision, void *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,10 @@ SIGNATURES = {
     "kf_local_full_matvec": [_vp] * 9 + [_c_int] * 5 + [_c_double, _vp, _vp, _vp],
     "kf_local_hessian_matvec": [_vp] * 8 + [_c_int] * 5 + [_c_double, _vp, _vp],
     "kf_local_jacobian_t_matvec": [_vp] * 7 + [_c_int] * 5 + [_c_double, _vp, _vp],
+    "kf_matvec_plan_create": [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int,
+                              _c_int, _c_double, ctypes.POINTER(_vp), _vp],
+    "kf_matvec_plan_execute": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "kf_matvec_plan_destroy": [_vp],
     "kf_local_full_diag_blocks": [_vp] * 4 + [_c_int] * 3 + [_c_double, _vp, _vp],
     "kf_block_factor": [_vp, _c_int, _c_int, _vp, _c_int, _c_double, _vp, _vp, _vp, _vp],
     "kf_block_apply": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp],
@@ -86,6 +90,7 @@ SIGNATURES = {
     "kf_cast_bf16": [_vp, _c_int, _vp, _vp],
     "kf_trttf": [_c_char, _c_char, _c_int, _vp, _c_int, _vp, _vp],
     "kf_tfttr": [_c_char, _c_char, _c_int, _vp, _vp, _c_int, _c_int, _vp],
+    "kf_rfp_relayout": [_c_char, _c_char, _c_char, _c_char, _c_int, _vp, _c_int, _vp, _vp],
 }
 _RESTYPE = {"kf_last_error": ctypes.c_char_p}
 

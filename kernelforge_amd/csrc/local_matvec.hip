@@ -18,6 +18,7 @@
 // then one pass over dX1 forms J_ai^T (G_ai - w_ai x_ai) per query atom, and per-
 // molecule sums in a fixed order (deterministic, no atomics).
 #include <algorithm>
+#include <new>
 #include <cmath>
 #include <vector>
 
@@ -190,6 +191,57 @@ __global__ __launch_bounds__(NT) void reduce_mol_kernel(const double *__restrict
         }
 }
 
+// backproj_kernel with the molecule's atom count taken from the force offsets
+// (3 N[a] = offs[a + 1] - offs[a]), so a plan needs no N array at execution.
+__global__ __launch_bounds__(NT) void backproj_plan_kernel(const double *__restrict__ G,
+                                                           const double *__restrict__ X1p,
+                                                           const double *__restrict__ wsum,
+                                                           const double *__restrict__ dX1,
+                                                           const int *__restrict__ qsrc,
+                                                           const int *__restrict__ offs, int rows,
+                                                           int ma, int rep,
+                                                           double *__restrict__ Fat) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int src = qsrc[r], a = src / ma;
+    const int nc = offs[a + 1] - offs[a], ld = 3 * ma;
+    const double *J = dX1 + (size_t)src * rep * ld;
+    const double *g = G + (size_t)r * rep, *x = X1p + (size_t)r * rep;
+    const double w = wsum[r];
+    double s0 = 0.0, s1 = 0.0;
+    for (int f = 0; f < rep; ++f) {
+        const double gv = g[f] - w * x[f];
+        const double *row = J + (size_t)f * ld;
+        if (lane < nc) s0 = fma(row[lane], gv, s0);
+        if (lane + 64 < nc) s1 = fma(row[lane + 64], gv, s1);
+    }
+    double *o = Fat + (size_t)src * ld;
+    if (lane < nc) o[lane] = s0;
+    if (lane + 64 < nc) o[lane + 64] = s1;
+}
+
+// reduce_mol_kernel with atom counts from the force offsets.
+__global__ __launch_bounds__(NT) void reduce_plan_kernel(const double *__restrict__ erow,
+                                                         const double *__restrict__ Fat,
+                                                         const int *__restrict__ offs, int nm,
+                                                         int ma, double *__restrict__ E,
+                                                         double *__restrict__ F) {
+    const int a = blockIdx.x;
+    if (a >= nm) return;
+    const int nc = offs[a + 1] - offs[a], na = nc / 3, ld = 3 * ma;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < na; ++i) s += erow[(size_t)a * ma + i];
+        E[a] = s;
+    }
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < na; ++i) s += Fat[((size_t)a * ma + i) * ld + c];
+        F[offs[a] + c] = s;
+    }
+}
+
 // Expand alpha_E (per molecule) to the packed train rows.
 __global__ void expand_mol(const double *__restrict__ aE, const int *__restrict__ src, int rows,
                            int ma, double *__restrict__ out) {
@@ -309,10 +361,208 @@ int matvec_impl(const double *X1, const double *dX1, const double *X2, const dou
     return KF_OK;
 }
 
+// Fill a buffer with zeros by a kernel (stream-ordered, and safe inside a captured graph).
+__global__ void zero_fill(double *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+}  // namespace
+}  // namespace kf
+
+// A full matvec planned once for fixed labels of the query set and a fixed training side
+// (descriptors, alpha_desc, alpha_E): the label plans, the packed training rows of every
+// label with their norms and dot products, and the query workspace all live in device
+// memory owned by the plan, so an execution neither reads anything back nor allocates.
+struct kf_matvec_plan {
+    int device = 0;
+    int nm1 = 0, ma1 = 0, nm2 = 0, ma2 = 0, rep = 0;
+    double sigma = 1.0;
+    struct Label {
+        int r1b, R1, R2, QB;
+        size_t train_off;  // doubles into `train`
+    };
+    std::vector<Label> labels;
+    int QBmax = 0;
+    size_t q_doubles = 0;  // per-execution query workspace
+    int *meta = nullptr;   // offs (nm1 + 1) | src1
+    double *train = nullptr, *query = nullptr, *acc = nullptr;  // acc: erow | Fat
+    size_t acc_doubles = 0;
+    ~kf_matvec_plan() {
+        (void)hipFree(meta);
+        (void)hipFree(train);
+        (void)hipFree(query);
+        (void)hipFree(acc);
+    }
+};
+
+namespace kf {
+namespace {
+
+int plan_alloc(void **p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 256) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        set_last_error("hipMalloc of %zu bytes for a matvec plan failed", bytes);
+        return KF_ERR_NOMEM;
+    }
+    return KF_OK;
+}
+
+int plan_create(const int *Q1, const int *N1, int nm1, int ma1, const double *X2,
+                const double *ad, const double *aE, const int *Q2, const int *N2, int nm2,
+                int ma2, int rep, double sigma, kf_matvec_plan *P, hipStream_t stream) {
+    KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
+    KF_REQUIRE(nm1 >= 1 && nm2 >= 0 && ma1 >= 1 && ma2 >= 0 && rep >= 1, KF_ERR_ARG,
+               "bad dimensions");
+    KF_REQUIRE(3 * ma1 <= MAXC, KF_ERR_BASIS, "matvec supports at most %d atoms per molecule",
+               MAXC / 3);
+    KF_REQUIRE(Q1 && N1 && (nm2 == 0 || (X2 && ad && aE && Q2 && N2)), KF_ERR_ARG,
+               "null pointer");
+    KF_HIP_TRY(hipGetDevice(&P->device));
+    P->nm1 = nm1; P->ma1 = ma1; P->nm2 = nm2; P->ma2 = ma2; P->rep = rep; P->sigma = sigma;
+    std::vector<int> q1, n1, q2, n2;
+    int rc;
+    if ((rc = read_metadata(Q1, N1, nm1, ma1, q1, n1, stream, WS_AUX0))) return rc;
+    if (nm2 > 0 && (rc = read_metadata(Q2, N2, nm2, ma2, q2, n2, stream, WS_AUX1))) return rc;
+    std::vector<int> labels = distinct_labels(q1.data(), n1.data(), nm1, ma1);
+    LocalPlan p1, p2;
+    build_plan(q1.data(), n1.data(), nm1, ma1, labels, p1);
+    if (nm2 > 0) build_plan(q2.data(), n2.data(), nm2, ma2, labels, p2);
+    std::vector<int> meta(nm1 + 1 + p1.src.size(), 0);
+    for (int a = 0; a < nm1; ++a) meta[a + 1] = meta[a] + 3 * clamp_atoms(n1[a], ma1);
+    std::copy(p1.src.begin(), p1.src.end(), meta.begin() + nm1 + 1);
+    std::vector<int> src2_all;
+    size_t train_doubles = 0;
+    for (int l = 0; l < p1.nlabels && nm2 > 0; ++l) {
+        const int r1b = p1.mol_row[(size_t)l * (nm1 + 1)];
+        const int R1 = p1.mol_row[(size_t)l * (nm1 + 1) + nm1] - r1b;
+        const int r2b = p2.mol_row[(size_t)l * (nm2 + 1)];
+        const int R2 = p2.mol_row[(size_t)l * (nm2 + 1) + nm2] - r2b;
+        if (R1 == 0 || R2 == 0) continue;
+        const size_t budget = size_t(1) << 30;
+        const int QB = (int)std::min<size_t>(
+            R1, std::max<size_t>(64, budget / (sizeof(double) * ((size_t)2 * R2 + 2 * rep + 2))));
+        P->labels.push_back({r1b, R1, R2, QB, train_doubles});
+        train_doubles += (size_t)2 * R2 * rep + 3 * (size_t)R2;
+        P->q_doubles = std::max(P->q_doubles,
+                                (size_t)QB * (2 * (size_t)R2 + 2 * (size_t)rep) + 2 * (size_t)QB);
+        P->QBmax = std::max(P->QBmax, QB);
+        src2_all.insert(src2_all.end(), p2.src.begin() + r2b, p2.src.begin() + r2b + R2);
+    }
+    const size_t natom1 = (size_t)nm1 * ma1;
+    P->acc_doubles = natom1 + natom1 * 3 * ma1;
+    if ((rc = plan_alloc((void **)&P->meta, sizeof(int) * (meta.size() + src2_all.size()))))
+        return rc;
+    if ((rc = plan_alloc((void **)&P->train, sizeof(double) * train_doubles))) return rc;
+    if ((rc = plan_alloc((void **)&P->query, sizeof(double) * P->q_doubles))) return rc;
+    if ((rc = plan_alloc((void **)&P->acc, sizeof(double) * P->acc_doubles))) return rc;
+    meta.insert(meta.end(), src2_all.begin(), src2_all.end());
+    KF_HIP_TRY(hipMemcpyAsync(P->meta, meta.data(), sizeof(int) * meta.size(),
+                              hipMemcpyHostToDevice, stream));
+    // pack the training side of every label once
+    const int *d_src2 = P->meta + nm1 + 1 + p1.src.size();
+    size_t s2 = 0;
+    for (const auto &L : P->labels) {
+        double *P2 = P->train + L.train_off;
+        double *AD2 = P2 + (size_t)L.R2 * rep;
+        double *nrm2 = AD2 + (size_t)L.R2 * rep, *rdot = nrm2 + L.R2, *aEr = rdot + L.R2;
+        hipLaunchKernelGGL(gather_rows, dim3((L.R2 + 3) / 4), dim3(NT), 0, stream, X2, ad,
+                           d_src2 + s2, L.R2, rep, P2, AD2, nrm2, rdot);
+        hipLaunchKernelGGL(expand_mol, dim3((L.R2 + 255) / 256), dim3(256), 0, stream, aE,
+                           d_src2 + s2, L.R2, ma2, aEr);
+        KF_HIP_TRY(hipGetLastError());
+        s2 += L.R2;
+    }
+    // the host vectors above are released on return: wait for the copy that reads them
+    KF_HIP_TRY(hipStreamSynchronize(stream));
+    return KF_OK;
+}
+
+int plan_execute(const kf_matvec_plan *P, const double *X1, const double *dX1, double *E,
+                 double *F, hipStream_t stream) {
+    KF_REQUIRE(P && X1 && dX1 && E && F, KF_ERR_ARG, "null pointer");
+    int dev = -1;
+    KF_HIP_TRY(hipGetDevice(&dev));
+    KF_REQUIRE(dev == P->device, KF_ERR_ARG, "plan was created on device %d, current is %d",
+               P->device, dev);
+    const int nm1 = P->nm1, ma1 = P->ma1, rep = P->rep;
+    const int *d_offs = P->meta, *d_src1 = P->meta + nm1 + 1;
+    const int *N1 = nullptr;  // atom counts are baked into the plan (offs, row lists)
+    double *erow = P->acc, *Fat = P->acc + (size_t)nm1 * ma1;
+    hipLaunchKernelGGL(zero_fill, dim3(256), dim3(256), 0, stream, P->acc, P->acc_doubles);
+    const double cexp = -1.0 / (2.0 * P->sigma * P->sigma), inv_s2 = 1.0 / (P->sigma * P->sigma);
+    int rc;
+    for (const auto &L : P->labels) {
+        const double *P2 = P->train + L.train_off;
+        const double *nrm2 = P2 + (size_t)2 * L.R2 * rep, *rdot = nrm2 + L.R2, *aEr = rdot + L.R2;
+        double *T = P->query;
+        double *Xq = T + (size_t)L.QB * 2 * L.R2, *G = Xq + (size_t)L.QB * rep;
+        double *nq = G + (size_t)L.QB * rep, *wsum = nq + L.QB;
+        for (int q0 = 0; q0 < L.R1; q0 += L.QB) {
+            const int qb = std::min(L.QB, L.R1 - q0);
+            const int *qsrc = d_src1 + L.r1b + q0;
+            hipLaunchKernelGGL(gather_rows, dim3((qb + 3) / 4), dim3(NT), 0, stream, X1,
+                               (const double *)nullptr, qsrc, qb, rep, Xq, (double *)nullptr, nq,
+                               (double *)nullptr);
+            if ((rc = gemm::gemm_f64<true>(qb, 2 * L.R2, rep, 1.0, Xq, rep, P2, rep, 0.0, T,
+                                           (size_t)2 * L.R2, stream)))
+                return rc;
+            hipLaunchKernelGGL(coef_kernel, dim3((qb + 3) / 4), dim3(NT), 0, stream, T, qb, L.R2,
+                               nq, nrm2, rdot, aEr, cexp, inv_s2, qsrc, erow, wsum);
+            if ((rc = gemm::gemm_f64<false>(qb, rep, 2 * L.R2, 1.0, T, (size_t)2 * L.R2, P2, rep,
+                                            0.0, G, rep, stream)))
+                return rc;
+            hipLaunchKernelGGL(backproj_plan_kernel, dim3((qb + 3) / 4), dim3(NT), 0, stream, G,
+                               Xq, wsum, dX1, qsrc, d_offs, qb, ma1, rep, Fat);
+            KF_HIP_TRY(hipGetLastError());
+        }
+    }
+    (void)N1;
+    hipLaunchKernelGGL(reduce_plan_kernel, dim3(nm1), dim3(128), 0, stream, erow, Fat, d_offs,
+                       nm1, ma1, E, F);
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
+}
+
 }  // namespace
 }  // namespace kf
 
 using namespace kf;
+
+extern "C" int kf_matvec_plan_create(const int *Q1, const int *N1, int nm1, int max_atoms1,
+                                     const double *X2, const double *alpha_desc,
+                                     const double *alpha_E, const int *Q2, const int *N2,
+                                     int nm2, int max_atoms2, int rep, double sigma,
+                                     kf_matvec_plan **plan, void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
+    KF_REQUIRE(plan, KF_ERR_ARG, "null plan out-pointer");
+    *plan = nullptr;
+    auto *P = new (std::nothrow) kf_matvec_plan();
+    KF_REQUIRE(P, KF_ERR_NOMEM, "plan allocation failed");
+    const int rc = plan_create(Q1, N1, nm1, max_atoms1, X2, alpha_desc, alpha_E, Q2, N2, nm2,
+                               max_atoms2, rep, sigma, P, as_stream(stream));
+    if (rc) {
+        delete P;
+        return rc;
+    }
+    *plan = P;
+    return KF_OK;
+}
+
+extern "C" int kf_matvec_plan_execute(const kf_matvec_plan *plan, const double *X1,
+                                      const double *dX1, double *E, double *F, void *stream) {
+    return plan_execute(plan, X1, dX1, E, F, as_stream(stream));
+}
+
+extern "C" int kf_matvec_plan_destroy(kf_matvec_plan *plan) {
+    if (plan) {
+        (void)hipDeviceSynchronize();  // no execution may still read the plan's memory
+        delete plan;
+    }
+    return KF_OK;
+}
 
 extern "C" int kf_local_alpha_desc(const double *dX, const int *Q, const int *N, int nm,
                                    int max_atoms, int rep, const double *alpha, double *out,

@@ -56,6 +56,20 @@ __global__ void tfttr_kernel(int n, bool trans, bool upper, const double *arf, d
     }
 }
 
+// Symmetric matrix in one RFP variant -> another (elements (i, j), i <= j, of the
+// column-major upper triangle; the lower-triangle variants store (j, i)).
+template <typename Tin, typename Tout>
+__global__ void relayout_kernel(int n, bool tin, bool uin, bool tout, bool uout,
+                                const Tin *__restrict__ in, Tout *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const int i = (int)(t % n), j = (int)(t / n);
+    if (i > j) return;
+    const size_t src = uin ? rfp_pos(n, tin, true, i, j) : rfp_pos(n, tin, false, j, i);
+    const size_t dst = uout ? rfp_pos(n, tout, true, i, j) : rfp_pos(n, tout, false, j, i);
+    out[dst] = (Tout)in[src];
+}
+
 inline int flags(char transr, char uplo, bool &trans, bool &upper) {
     KF_REQUIRE(transr == 'N' || transr == 'n' || transr == 'T' || transr == 't', KF_ERR_ARG,
                "transr must be 'N' or 'T'");
@@ -99,6 +113,37 @@ extern "C" int kf_tfttr(char transr, char uplo, int n, const double *arf, double
     const size_t tot = (size_t)n * n;
     hipLaunchKernelGGL(tfttr_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        as_stream(stream), n, trans, upper, arf, A, lda, symmetrize);
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
+}
+
+// Symmetric RFP re-layout: a matrix stored as RFP (transr_in, uplo_in) rewritten as
+// (transr_out, uplo_out), optionally narrowed from FP64 to FP32 (out_precision
+// KF_PREC_F32); in and out must not overlap.  Used to hand the kernels' native packing
+// (TRANSR='N', UPLO='U', the reference CPU kernels' own) to callers of the reference's
+// GPU kernels, which pack TRANSR='N', UPLO='L' (src/cuda_local_kernels.hpp:52).
+extern "C" int kf_rfp_relayout(char transr_in, char uplo_in, char transr_out, char uplo_out,
+                               int n, const double *in, int out_precision, void *out,
+                               void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
+    using namespace kf;
+    bool tin, uin, tout, uout;
+    int rc = flags(transr_in, uplo_in, tin, uin);
+    if (rc) return rc;
+    if ((rc = flags(transr_out, uplo_out, tout, uout))) return rc;
+    KF_REQUIRE(out_precision == KF_PREC_F64 || out_precision == KF_PREC_F32, KF_ERR_ARG,
+               "out_precision must be F64 or F32");
+    KF_REQUIRE(n >= 0, KF_ERR_ARG, "n must be >= 0");
+    if (n == 0) return KF_OK;
+    KF_REQUIRE(in && out && in != out, KF_ERR_ARG, "null or aliased pointer");
+    const size_t tot = (size_t)n * n;
+    const dim3 grid((unsigned)((tot + 255) / 256));
+    if (out_precision == KF_PREC_F64)
+        hipLaunchKernelGGL((relayout_kernel<double, double>), grid, dim3(256), 0, as_stream(stream), n,
+                           tin, uin, tout, uout, in, static_cast<double *>(out));
+    else
+        hipLaunchKernelGGL((relayout_kernel<double, float>), grid, dim3(256), 0, as_stream(stream), n,
+                           tin, uin, tout, uout, in, static_cast<float *>(out));
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }

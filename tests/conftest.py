@@ -19,6 +19,8 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP path vs the CPU oracle)")
     config.addinivalue_line("markers", "slow: long-running")
+    config.addinivalue_line("markers", "cpu_only: passes placeholder pointers; skipped where a "
+                            "GPU is visible")
 
 
 @pytest.fixture(scope="session")

@@ -69,6 +69,8 @@ def test_pcg_entry_points_reject_block_size_before_any_launch():
     from kernelforge_amd import capi
 
     L = capi.lib()
+    if L.kf_device_name(None, 0) == capi.KF_OK:
+        pytest.skip("placeholder device pointers: CPU only")
     fake = 4096  # never dereferenced: the argument check comes first
     it, rel = ctypes.c_int(0), ctypes.c_double(0.0)
     loss = ctypes.c_double(0.0)
