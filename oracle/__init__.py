@@ -199,6 +199,35 @@ def ref_fchl19(coords, z, grad=False, **kw):
     return (rep, g) if grad else rep
 
 
+def ref_fchl19v2(coords, z, grad=False, elements=(1, 6, 7, 8, 16), **kw):
+    """The reference's own fchl19v2_repr.generate[_and_gradients] (oracle/_ref), or None.
+    Keywords as oracle.fchl19v2.params (two_body_type / three_body_type by name)."""
+    from . import fchl19v2 as v2
+
+    R = ref()
+    if R is None:
+        return None
+    p = v2.params(**kw)
+    c = _f64(coords).reshape(-1)
+    z = _i32(z)
+    n = z.shape[0]
+    el = _i32(elements)
+    nab = v2.nabasis(p["three_body"], p["nFourier"], p["nCosine"])
+    rs = v2.rep_size(len(el), p["nRs2"], p["nRs3"], nab, p["three_body"], p["nRs3_minus"])
+    rep = np.zeros((n, rs))
+    g = np.zeros((n, rs, 3 * n)) if grad else None
+    dbl = ctypes.c_double
+    rc = R.kfr_fchl19v2(_d(c), _i(z), n, _i(el), len(el), p["nRs2"], p["nRs3"], p["nRs3_minus"],
+                        p["nFourier"], p["nCosine"], dbl(p["eta2"]), dbl(p["eta3"]),
+                        dbl(p["eta3_minus"]), dbl(p["zeta"]), dbl(p["rcut"]), dbl(p["acut"]),
+                        dbl(p["two_body_decay"]), dbl(p["three_body_decay"]),
+                        dbl(p["three_body_weight"]), v2.TWO_BODY.index(p["two_body"]),
+                        v2.THREE_BODY.index(p["three_body"]), _d(rep), _d(g))
+    if rc:
+        raise ValueError("reference fchl19v2 raised")
+    return (rep, g) if grad else rep
+
+
 def ref_cho_solve_rfp(arf, y, l2=0.0):
     R = ref()
     if R is None:

@@ -10,6 +10,10 @@ Fixtures (inputs and the reference's outputs only):
                   10 molecules; generate_fchl_acsf_and_gradients (rep, grad) for the 3
                   smallest; plus one molecule under non-default hyper-parameters.
   rfp_ref.npz     a 57x57 SPD system in RFP with the reference cho_solve_rfp result.
+  fchl19v2_ref.npz  FCHL19v2 (src/fchl19v2_repr.cpp built into oracle/_ref): for every
+                  two-body x three-body basis pair, generate() of a 4-atom H/C/O molecule
+                  under small grids; generate_and_gradients() for one pair per three-body
+                  basis.
 """
 
 import os
@@ -67,7 +71,34 @@ def main():
     y = rng.standard_normal(n)
     alpha = oracle.ref_cho_solve_rfp(arf, y, 1e-3)
     np.savez_compressed(os.path.join(HERE, "rfp_ref.npz"), arf=arf, y=y, l2=1e-3, alpha=alpha)
+
+    from oracle import fchl19v2 as v2
+
+    v2_out = {}
+    mol = np.array([[0.0, 0.0, 0.0], [1.21, 0.08, -0.05], [-0.42, 1.02, 0.11],
+                    [0.35, -0.31, 1.07]])
+    z = np.array([6, 8, 1, 1], dtype=np.int32)
+    v2_out["coords"], v2_out["z"], v2_out["elements"] = mol, z, np.array(V2_ELEMENTS)
+    for ti, tb in enumerate(v2.TWO_BODY):
+        for ai, ab in enumerate(v2.THREE_BODY):
+            kw = v2_golden_kw(tb, ab)
+            v2_out[f"rep_{tb}_{ab}"] = oracle.ref_fchl19v2(mol, z, elements=V2_ELEMENTS, **kw)
+            if ti == ai:
+                r, g = oracle.ref_fchl19v2(mol, z, grad=True, elements=V2_ELEMENTS, **kw)
+                v2_out[f"grep_{tb}_{ab}"], v2_out[f"ggrad_{tb}_{ab}"] = r, g
+    np.savez_compressed(os.path.join(HERE, "fchl19v2_ref.npz"), **v2_out)
     print("wrote", os.listdir(HERE))
+
+
+V2_ELEMENTS = [1, 6, 8]
+
+
+def v2_golden_kw(tb, ab):
+    """Small grids of the FCHL19v2 fixtures (nabasis 4 for both angular families)."""
+    return dict(two_body_type=tb, three_body_type=ab, nRs2=6, nRs3=4,
+                nRs3_minus=3 if ab in ("odd_fourier_split_r", "cosine_split_r",
+                                       "cosine_split_r_no_atm") else 0,
+                nFourier=2, nCosine=4, eta2=0.4, eta3=2.3, eta3_minus=1.9, rcut=6.0, acut=5.0)
 
 
 if __name__ == "__main__":
