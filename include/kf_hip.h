@@ -317,6 +317,19 @@ int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *stream);
  * double b: the solve accumulates in FP64). */
 int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream);
 
+/* kf_rfp_potrs without allocation: the same solve, with its scratch (the 64x64 inverses
+ * of the diagonal blocks and the intermediate L^-1 b) in a fixed 20 MiB device array
+ * of the library instead of allocated workspace.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under graph capture, tight serving
+ * loops that solve many right-hand sides with one factor).  Orders up to 32,768 with one
+ * right-hand side fit (the scratch needs 8 * 4096 * nblk + 8 * n * nrhs bytes, nblk =
+ * ceil((n/2)/64) + ceil((n - n/2)/64)); larger requests return KF_ERR_ARG.  The first
+ * call on a device looks up the array's device address; no call allocates.  Calls on
+ * one device share the array: calls that overlap on one device race (the library's
+ * per-device serialisation of entry points orders calls on different streams). */
+int kf_rfp_potrs_noalloc(int precision, const void *arf, int n, double *b, int nrhs,
+                         void *stream);
+
 /* y = (A + lambda I) x for a symmetric RFP matrix A (precision KF_PREC_F64: double*,
  * KF_PREC_F32: float*), FP64 vectors and accumulation, deterministic.  The operator
  * of the refinement residual r = y - (K + lambda I) alpha (no reference counterpart:

@@ -50,6 +50,7 @@ SIGNATURES = {
                         _c_int, _c_double, _vp, _vp],
     "kf_rfp_potrf": [_c_int, _vp, _c_int, _c_double, _vp],
     "kf_rfp_potrs": [_c_int, _vp, _c_int, _vp, _c_int, _vp],
+    "kf_rfp_potrs_noalloc": [_c_int, _vp, _c_int, _vp, _c_int, _vp],
     "kf_rfp_symv": [_c_int, _vp, _c_int, _c_double, _vp, _vp, _vp],
     "kf_rfp_pcg": [_c_int, _vp, _c_int, _vp, _c_int, _c_double, _vp, _vp, _c_double, _c_int,
                    ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp],

@@ -935,6 +935,8 @@ int potrs_run(const T *a, const Part &P, double *b, int nrhs, char *scratch, hip
     const int n = P.n;
     double *inv = reinterpret_cast<double *>(scratch);
     double *y = reinterpret_cast<double *>(scratch + potrs_inv_bytes(P));
+    // the kernel timer creates HIP events when enabled, so the allocation-free entry
+    // point runs untimed
     ScopedTimer timer(timed ? TM_POTRS : -1, stream);
     hipLaunchKernelGGL(inv_diag_all<T>, dim3(P.nblk), dim3(256), 0, stream, a, P, inv);
     for (int J = 0; J < P.nblk; ++J)
@@ -956,6 +958,33 @@ int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
     return potrs_run(a, P, b, nrhs, ws, stream, true);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free solve: the same kernels with their scratch in a fixed device array.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(20) << 20;  // 20 MiB per device
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDev = 16;
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[kMaxDev] = {};
+
+// Device address of `scratch` on the current device, looked up once per device (the
+// host-side name of a __device__ array is only a stand-in; kernels must be given this).
+int scratch_address(char **out) {
+    int dev = 0;
+    KF_HIP_TRY(hipGetDevice(&dev));
+    KF_REQUIRE(dev >= 0 && dev < kMaxDev, KF_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        KF_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        KF_REQUIRE(p, KF_ERR_HIP, "scratch symbol has no device address");
+        g_scratch_ptr[dev] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return KF_OK;
+}
+
 }  // namespace
 }  // namespace kf
 
@@ -966,6 +995,28 @@ extern "C" int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *st
     if (precision == KF_PREC_F32) return potrf_impl(static_cast<float *>(arf), n, l2, as_stream(stream));
     set_last_error("precision %d not supported by kf_rfp_potrf", precision);
     return KF_ERR_ARG;
+}
+
+extern "C" int kf_rfp_potrs_noalloc(int precision, const void *arf, int n, double *b, int nrhs,
+                                    void *stream) {
+    kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
+    using namespace kf;
+    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_F32, KF_ERR_ARG,
+               "precision %d not supported by kf_rfp_potrs_noalloc", precision);
+    KF_REQUIRE(arf && b && n > 0 && nrhs >= 1, KF_ERR_ARG, "bad potrs arguments");
+    const Part P = make_part(n);
+    const size_t need = potrs_scratch_bytes(P, n, nrhs);
+    KF_REQUIRE(need <= kNoAllocScratchBytes, KF_ERR_ARG,
+               "kf_rfp_potrs_noalloc: order %d with %d right-hand sides needs %zu bytes of "
+               "scratch, more than its fixed %zu (use kf_rfp_potrs)",
+               n, nrhs, need, kNoAllocScratchBytes);
+    char *scr = nullptr;
+    int rc = scratch_address(&scr);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    if (precision == KF_PREC_F64)
+        return potrs_run(static_cast<const double *>(arf), P, b, nrhs, scr, st, false);
+    return potrs_run(static_cast<const float *>(arf), P, b, nrhs, scr, st, false);
 }
 
 extern "C" int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream) {
