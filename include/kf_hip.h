@@ -99,6 +99,24 @@ int kf_fchl19_f64(const double *coords, const int *Q, const int *N, int nm, int 
                   double acut, double two_body_decay, double three_body_decay, double w3,
                   double *X, double *dX, void *stream);
 
+/* FCHL19v2 representation (and optionally its Jacobian) of a padded batch, with
+ * selectable bases.  Replaces kf::fchl19v2::generate (src/fchl19v2_repr.cpp:2874) and
+ * generate_and_gradients (:2919).  two_body_type / three_body_type index the reference's
+ * TwoBodyType {log_normal, gaussian_r, gaussian_log_r, gaussian_r_no_pow, bessel} and
+ * ThreeBodyType {odd_fourier_rbar, cosine_rbar, odd_fourier_split_r, cosine_split_r,
+ * cosine_split_r_no_atm} (src/fchl19v2_repr.hpp:11-27).  elements / Rs2 / Rs3 /
+ * Rs3_minus are HOST arrays; w3 = sqrt(eta3/pi) * three_body_weight; Rs3_minus is used
+ * by the split bases only (nRs3_minus > 0 there).  Layouts as kf_fchl19_f64, with
+ * rep = nelem nRs2 + nelem(nelem+1)/2 nRs3 [nRs3_minus] nabasis.  No limit on
+ * max_atoms; up to 16 elements and 64 functions per radial grid. */
+int kf_fchl19v2_f64(const double *coords, const int *Q, const int *N, int nm, int max_atoms,
+                    const int *elements, int nelem, const double *Rs2, int nRs2,
+                    const double *Rs3, int nRs3, const double *Rs3_minus, int nRs3_minus,
+                    int nabasis, int two_body_type, int three_body_type, double eta2,
+                    double eta3, double eta3_minus, double zeta, double rcut, double acut,
+                    double two_body_decay, double three_body_decay, double w3, double *X,
+                    double *dX, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Local (element-matched) Gaussian kernels                                 */
 /*   K[a,b] = sum_{i in a, j in b, Q_i == Q_j} exp(-|x_i - x_j|^2 / 2 sigma^2) */
@@ -125,8 +143,9 @@ int kf_local_kernel(int precision, const double *X1, const double *X2, const int
 /* ------------------------------------------------------------------------ */
 /* Energy+force forms.  dX arrays are (nm, max_atoms, rep, 3*max_atoms): the  */
 /* Jacobian of each atom's descriptor with respect to its molecule's         */
-/* coordinates; naq = 3 * sum(clamped N).  max_atoms <= 32 for these forms   */
-/* (KF_ERR_BASIS otherwise).  All arrays are device pointers.                */
+/* coordinates; naq = 3 * sum(clamped N).  Any max_atoms: up to 32 atoms per */
+/* molecule run the MFMA kernels, larger molecules a size-general kernel.    */
+/* All arrays are device pointers.                                           */
 /* ------------------------------------------------------------------------ */
 
 /* (nm1, naq2) row-major, +dK/dR2.  Replaces kf::fchl19::kernel_gaussian_jacobian

@@ -30,9 +30,10 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["kf_runtime.hip", "fchl19.hip", "local_scalar.hip", "local_mixed.hip", "rfp_chol.hip",
                "rfp_convert.hip", "rfp_solve.hip", "local_deriv.hip",
-               "local_matvec.hip", "ef_pcg.hip", "mlp.hip", "feature_compact.hip"]
+               "local_matvec.hip", "ef_pcg.hip", "mlp.hip", "feature_compact.hip",
+               "fchl19v2.hip"]
 PY_MODULES = {"fchl19_repr": "py_fchl19_repr.cpp", "local_kernels": "py_local_kernels.cpp",
-              "kernelmath": "py_kernelmath.cpp"}
+              "kernelmath": "py_kernelmath.cpp", "fchl19v2_repr": "py_fchl19v2_repr.cpp"}
 
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -43,6 +43,8 @@ SIGNATURES = {
     "kf_fchl19_f64": [_vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int,
                       _c_int, _c_double, _c_double, _c_double, _c_double, _c_double, _c_double,
                       _c_double, _c_double, _vp, _vp, _vp],
+    "kf_fchl19v2_f64": [_vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int,
+                        _vp, _c_int, _c_int, _c_int, _c_int] + [_c_double] * 11 + [_vp, _vp, _vp],
     "kf_local_kernel_symm_rfp": [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_double, _vp,
                                  _vp],
     "kf_local_kernel_symm": [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp],

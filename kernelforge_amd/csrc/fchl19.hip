@@ -506,8 +506,7 @@ int build_params(FchlParams &P, int max_atoms, const int *elements, int nelem, c
                "nRs2/nRs3 must be in [1, %d]", kMaxRs);
     KF_REQUIRE(nabasis >= 0 && nabasis % 2 == 0 && nabasis <= 2 * kMaxHarm, KF_ERR_BASIS,
                "Ts.size() (nabasis) must be even and <= %d", 2 * kMaxHarm);
-    KF_REQUIRE(max_atoms >= 1 && max_atoms <= kMaxAtoms, KF_ERR_ARG,
-               "max_atoms must be in [1, %d]", kMaxAtoms);
+    KF_REQUIRE(max_atoms >= 1, KF_ERR_ARG, "max_atoms must be >= 1");
     for (int k = 0; k < nRs2; ++k)
         KF_REQUIRE(Rs2[k] > 0.0, KF_ERR_BASIS, "All Rs2 must be > 0");
     P.nelem = nelem;
@@ -560,30 +559,34 @@ extern "C" int kf_fchl19_f64(const double *coords, const int *Q, const int *N, i
                           eta3, zeta, rcut, acut, two_body_decay, three_body_decay, w3,
                           dX != nullptr);
     if (rc) return rc;
+    // LDS of the kernels below (whole molecules staged per workgroup)
+    const int ma = max_atoms, ld = 3 * ma;
+    const int ntri_cap = ma * (ma - 1) / 2;
+    const size_t lds_rep = sizeof(double) * (3 * ma) + sizeof(int) * ((ma + 1) & ~1) +
+                           sizeof(double) * kRepWaves * (64 * (3 + P.na) + P.nchan * P.nb3 * P.na);
+    const size_t lds_grad = sizeof(double) * 3 * ma + sizeof(int) * ((ma + 1) & ~1) +
+                            sizeof(int) * 2 * ((ntri_cap + 1) & ~1) +
+                            sizeof(double) * (size_t)ntri_cap * kRecW +
+                            sizeof(double) * (size_t)64 * ld +
+                            sizeof(int) * (((P.nchan + 2) & ~1) + ntri_cap);
+    const size_t lds = dX ? lds_grad : lds_rep;
+    if (ma > kMaxAtoms || lds > 160 * 1024) {
+        // Large molecules: the FCHL19v2 kernel with the log-normal two-body and odd-Fourier
+        // r_bar three-body bases is this representation (same values and Jacobian, the
+        // gradient path's first-harmonic convention included) and stages no molecule in LDS.
+        return kf_fchl19v2_f64(coords, Q, N, nm, max_atoms, elements, nelem, Rs2, nRs2, Rs3,
+                               nRs3, nullptr, 0, nabasis, 0, 0, eta2, eta3, eta3, zeta, rcut,
+                               acut, two_body_decay, three_body_decay, w3, X, dX, stream_);
+    }
     int *err = static_cast<int *>(workspace(WS_AUX3, 64, stream));
     KF_REQUIRE(err, KF_ERR_NOMEM, "workspace allocation failed");
     KF_HIP_TRY(hipMemsetAsync(err, 0, sizeof(int), stream));
     ScopedTimer timer(TM_FCHL19, stream);
     if (!dX) {
-        const int rec_w = 3 + P.na;
-        const int acc_n = P.nchan * P.nb3 * P.na;
-        const size_t lds = sizeof(double) * (3 * max_atoms) + sizeof(int) * ((max_atoms + 1) & ~1) +
-                           sizeof(double) * kRepWaves * (64 * rec_w + acc_n);
-        KF_REQUIRE(lds <= 160 * 1024, KF_ERR_BASIS, "basis too large for one workgroup (%zu B)",
-                   lds);
         hipLaunchKernelGGL(fchl19_rep_kernel, dim3(nm), dim3(kRepThreads), lds, stream, coords, Q,
                            N, P, X, err);
     } else {
         KF_REQUIRE(P.nelem * P.nb2 > 0, KF_ERR_BASIS, "empty two-body basis");
-        const int ma = max_atoms, ld = 3 * ma;
-        const int ntri_cap = ma * (ma - 1) / 2;
-        const size_t lds = sizeof(double) * 3 * ma + sizeof(int) * ((ma + 1) & ~1) +
-                           sizeof(int) * 2 * ((ntri_cap + 1) & ~1) +
-                           sizeof(double) * (size_t)ntri_cap * kRecW +
-                           sizeof(double) * (size_t)64 * ld +
-                           sizeof(int) * (((P.nchan + 2) & ~1) + ntri_cap);
-        KF_REQUIRE(lds <= 160 * 1024, KF_ERR_ARG,
-                   "max_atoms %d too large for the Jacobian kernel (%zu B LDS)", ma, lds);
         hipLaunchKernelGGL(fchl19_grad_kernel, dim3(ma, nm), dim3(kGradThreads), lds, stream,
                            coords, Q, N, P, X, dX, err);
     }

@@ -983,6 +983,198 @@ int deriv_launch(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1
     return KF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Any molecule size: the derivative forms for sets with more than MAXA atoms per molecule.
+// One work-group per molecule pair, every pair-size-dependent list in global scratch:
+//   matched list (i, j) of equal labels, i-major; e_m = exp(-|d_m|^2 / 2 s^2),
+//   d_m = x_ai - x_bj;  v_m = J_ai^T d_m (3 na), u_m = J_bj^T d_m (3 nb);
+//   EE = sum e_m, FE = -sum c_m v_m, EF = sum c_m u_m (c = e / s^2), and
+//   FF = sum_f J_a[:, f, :]^T T_f - sum_m (e_m / s^4) v_m u_m^T with
+//   T_f[i, :] = sum_{m in row i} c_m J_b[j_m, f, :],
+// FF in 64 x 64 output tiles, T_f staged in LDS per (feature, 64-row block of i).  The
+// blocks land through the same Out<MODE> addressing as the fixed-size kernels.  VALU
+// FP64: it carries the sizes the MFMA kernels above do not take, not their speed.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;  // output tile edge / staged i-rows
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void deriv_generic_kernel(DerivArgs A, int pair0,
+                                                           int *__restrict__ iscr,
+                                                           size_t iscr_len) {
+    __shared__ double T[GT][GT + 1];
+    __shared__ int s_np;
+    const int tid = threadIdx.x;
+    const int t = pair0 + blockIdx.x;
+    const int a = A.pairs[t].x, b = A.pairs[t].y;
+    const int na = clampi(A.n1[a], 0, A.ma1), nb = clampi(A.n2[b], 0, A.ma2);
+    const int ld1 = 3 * A.ma1, ld2 = 3 * A.ma2, rep = A.rep;
+    double *slot = A.slot + (size_t)blockIdx.x * A.slot_len;
+    double *em = slot, *V = slot + A.maxm, *U = V + (size_t)A.maxm * A.ldv;
+    int *mi = iscr + (size_t)blockIdx.x * iscr_len, *mj = mi + A.maxm, *ist = mj + A.maxm;
+    if (tid == 0) {  // matched list, i-major
+        int cnt = 0;
+        for (int i = 0; i < na; ++i) {
+            ist[i] = cnt;
+            const int qi = A.q1[(size_t)a * A.ma1 + i];
+            for (int j = 0; j < nb; ++j)
+                if (A.q2[(size_t)b * A.ma2 + j] == qi) { mi[cnt] = i; mj[cnt] = j; ++cnt; }
+        }
+        ist[na] = cnt;
+        s_np = cnt;
+    }
+    __syncthreads();
+    const int np = s_np;
+    const double *x1 = A.x1 + (size_t)a * A.ma1 * rep, *x2 = A.x2 + (size_t)b * A.ma2 * rep;
+    for (int m = tid; m < np; m += NT) {
+        const double *p = x1 + (size_t)mi[m] * rep, *q = x2 + (size_t)mj[m] * rep;
+        double s2 = 0.0;
+        for (int f = 0; f < rep; ++f) {
+            const double d = p[f] - q[f];
+            s2 = fma(d, d, s2);
+        }
+        em[m] = exp(A.cexp * s2);
+    }
+    const bool want_v = has_fe(MODE) || has_ff(MODE), want_u = has_ef(MODE) || has_ff(MODE);
+    if (want_v)
+        for (long idx = tid; idx < (long)np * 3 * na; idx += NT) {
+            const int m = (int)(idx / (3 * na)), r = (int)(idx - (long)m * 3 * na);
+            const double *p = x1 + (size_t)mi[m] * rep, *q = x2 + (size_t)mj[m] * rep;
+            const double *J = A.dx1 + ((size_t)a * A.ma1 + mi[m]) * rep * ld1 + r;
+            double s = 0.0;
+            for (int f = 0; f < rep; ++f) s = fma(J[(size_t)f * ld1], p[f] - q[f], s);
+            V[(size_t)m * A.ldv + r] = s;
+        }
+    if (want_u)
+        for (long idx = tid; idx < (long)np * 3 * nb; idx += NT) {
+            const int m = (int)(idx / (3 * nb)), c = (int)(idx - (long)m * 3 * nb);
+            const double *p = x1 + (size_t)mi[m] * rep, *q = x2 + (size_t)mj[m] * rep;
+            const double *J = A.dx2 + ((size_t)b * A.ma2 + mj[m]) * rep * ld2 + c;
+            double s = 0.0;
+            for (int f = 0; f < rep; ++f) s = fma(J[(size_t)f * ld2], p[f] - q[f], s);
+            U[(size_t)m * A.ldu + c] = s;
+        }
+    __syncthreads();
+    Out<MODE> out{A.out, A.ld, a, b, A.offs1[a], A.offs2[b], A.ebase1, A.ebase2};
+    if (has_ee(MODE) && tid == 0) {
+        double e = 0.0;
+        for (int m = 0; m < np; ++m) e += em[m];
+        out.ee(e);
+    }
+    if (has_fe(MODE))
+        for (int r = tid; r < 3 * na; r += NT) {
+            double s = 0.0;
+            for (int m = 0; m < np; ++m) s -= em[m] * A.inv_s2 * V[(size_t)m * A.ldv + r];
+            out.fe(r, s);
+        }
+    if (has_ef(MODE))
+        for (int c = tid; c < 3 * nb; c += NT) {
+            double s = 0.0;
+            for (int m = 0; m < np; ++m) s += em[m] * A.inv_s2 * U[(size_t)m * A.ldu + c];
+            out.ef(c, s);
+        }
+    if (!has_ff(MODE) || na == 0 || nb == 0) return;
+    const int tr = tid >> 4, tc = tid & 15;
+    const double *J1 = A.dx1 + (size_t)a * A.ma1 * rep * ld1;
+    const double *J2 = A.dx2 + (size_t)b * A.ma2 * rep * ld2;
+    for (int r0 = 0; r0 < 3 * na; r0 += GT)
+        for (int c0 = 0; c0 < 3 * nb; c0 += GT) {
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) acc[u][w] = 0.0;
+            for (int f = 0; f < rep; ++f)
+                for (int i0 = 0; i0 < na; i0 += GT) {
+                    __syncthreads();  // the previous T tile is no longer read
+                    for (int e = tid; e < GT * GT; e += NT) {
+                        const int ii = e / GT, cc = e - ii * GT;
+                        const int i = i0 + ii, c = c0 + cc;
+                        double s = 0.0;
+                        if (i < na && c < 3 * nb)
+                            for (int m = ist[i]; m < ist[i + 1]; ++m)
+                                s = fma(em[m] * A.inv_s2,
+                                        J2[((size_t)mj[m] * rep + f) * ld2 + c], s);
+                        T[ii][cc] = s;
+                    }
+                    __syncthreads();
+                    const int iend = min(GT, na - i0);
+                    for (int ii = 0; ii < iend; ++ii) {
+                        const double *jr = J1 + ((size_t)(i0 + ii) * rep + f) * ld1;
+                        double jv[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int r = r0 + tr + 16 * u;
+                            jv[u] = r < 3 * na ? jr[r] : 0.0;
+                        }
+#pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const double tv = T[ii][tc + 16 * w];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) acc[u][w] = fma(jv[u], tv, acc[u][w]);
+                        }
+                    }
+                }
+            // rank-one terms, then write the tile
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const int r = r0 + tr + 16 * u, c = c0 + tc + 16 * w;
+                    if (r >= 3 * na || c >= 3 * nb) continue;
+                    double s = acc[u][w];
+                    for (int m = 0; m < np; ++m)
+                        s -= em[m] * A.inv_s4 * V[(size_t)m * A.ldv + r] * U[(size_t)m * A.ldu + c];
+                    out.ff(r, c, s);
+                }
+        }
+}
+
+template <int MODE>
+int deriv_launch_generic(DerivArgs A, const std::vector<int2> &pairs, const SideMeta &s1,
+                         const SideMeta &s2, int nlab, hipStream_t stream) {
+    const size_t npairs = pairs.size();
+    if (npairs == 0) return KF_OK;
+    const size_t n_off = s1.offs.size() + s2.offs.size();
+    const size_t pair_off = (sizeof(int) * n_off + 15) / 16 * 16;
+    const size_t bytes = pair_off + sizeof(int2) * npairs;
+    char *dev = static_cast<char *>(workspace(WS_PLAN, bytes, stream));
+    char *stage = static_cast<char *>(host_staging(WS_PLAN, bytes));
+    KF_REQUIRE(dev && stage, KF_ERR_NOMEM, "workspace allocation failed (%zu B)", bytes);
+    std::copy(s1.offs.begin(), s1.offs.end(), reinterpret_cast<int *>(stage));
+    std::copy(s2.offs.begin(), s2.offs.end(), reinterpret_cast<int *>(stage) + s1.offs.size());
+    std::copy(pairs.begin(), pairs.end(), reinterpret_cast<int2 *>(stage + pair_off));
+    KF_HIP_TRY(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, stream));
+    A.offs1 = reinterpret_cast<int *>(dev);
+    A.offs2 = A.offs1 + s1.offs.size();
+    A.pairs = reinterpret_cast<const int2 *>(dev + pair_off);
+    int maxm = 1;
+    for (const int2 &p : pairs) {
+        int c = 0;
+        for (int l = 0; l < nlab; ++l)
+            c += s1.hist[(size_t)p.x * nlab + l] * s2.hist[(size_t)p.y * nlab + l];
+        maxm = std::max(maxm, c);
+    }
+    A.maxm = maxm;
+    A.ldv = 3 * A.ma1;
+    A.ldu = 3 * A.ma2;
+    A.slot_len = (size_t)maxm * (1 + A.ldv + A.ldu);
+    const size_t iscr_len = 2 * (size_t)maxm + A.ma1 + 1;
+    const size_t per_pair = A.slot_len * sizeof(double) + iscr_len * sizeof(int);
+    const size_t budget = size_t(1) << 30;
+    const size_t batch = std::max<size_t>(1, std::min(npairs, budget / per_pair));
+    char *ws = static_cast<char *>(workspace(WS_AUX2, batch * per_pair + 256, stream));
+    KF_REQUIRE(ws, KF_ERR_NOMEM, "scratch allocation failed");
+    A.slot = reinterpret_cast<double *>(ws);
+    int *iscr = reinterpret_cast<int *>(ws + batch * A.slot_len * sizeof(double));
+    for (size_t p0 = 0; p0 < npairs; p0 += batch) {
+        const unsigned nb = (unsigned)std::min(batch, npairs - p0);
+        hipLaunchKernelGGL(deriv_generic_kernel<MODE>, dim3(nb), dim3(NT), 0, stream, A, (int)p0,
+                           iscr, iscr_len);
+        KF_HIP_TRY(hipGetLastError());
+    }
+    return KF_OK;
+}
+
 // Drop the descriptor columns that are zero for every atom of both sets (in X and in the
 // Jacobians the mode reads): A then points at compacted copies.
 int compact_features(DerivArgs &A, const int *N1, const int *N2, int nm1, int nm2,
@@ -1005,9 +1197,8 @@ int deriv_impl(const double *X1, const double *dX1, const double *X2, const doub
     KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
     KF_REQUIRE(nm1 >= 0 && nm2 >= 0 && ma1 >= 0 && ma2 >= 0 && rep >= 1, KF_ERR_ARG,
                "bad dimensions");
-    KF_REQUIRE(ma1 <= MAXA && ma2 <= MAXA, KF_ERR_BASIS,
-               "derivative kernels support at most %d atoms per molecule (got %d, %d)", MAXA,
-               ma1, ma2);
+    // more than MAXA atoms per molecule: the size-general kernel
+    const bool general = ma1 > MAXA || ma2 > MAXA;
     if (nm1 == 0 || nm2 == 0) return KF_OK;
     KF_REQUIRE(X1 && X2 && Q1 && Q2 && N1 && N2 && out, KF_ERR_ARG, "null pointer");
     const bool need1 = has_fe(MODE) || has_ff(MODE), need2 = has_ef(MODE) || has_ff(MODE);
@@ -1071,6 +1262,7 @@ int deriv_impl(const double *X1, const double *dX1, const double *X2, const doub
         return KF_OK;
     if ((rc = compact_features(A, N1, N2, nm1, nm2, stream))) return rc;
     ScopedTimer timer(TM_FULL, stream);
+    if (general) return deriv_launch_generic<MODE>(A, pairs, s1, s2, (int)labels.size(), stream);
     return deriv_launch<MODE>(A, pairs, s1, s2, (int)labels.size(), stream);
 }
 

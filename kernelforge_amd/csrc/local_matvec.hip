@@ -30,7 +30,6 @@ namespace kf {
 namespace {
 
 constexpr int NT = 256;
-constexpr int MAXC = 128;  // Jacobian columns handled per lane pair (3 * max_atoms <= 128)
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -57,15 +56,18 @@ __global__ __launch_bounds__(NT) void alpha_desc_kernel(const double *__restrict
         return;
     }
     const int ld = 3 * ma, nc = 3 * nb;
-    const double a0 = lane < nc ? alpha[offs[b] + lane] : 0.0;
-    const double a1 = lane + 64 < nc ? alpha[offs[b] + lane + 64] : 0.0;
     const double *J = dX + atom * rep * ld;
-    for (int f = 0; f < rep; ++f) {
-        const double *row = J + (size_t)f * ld;
-        double s = lane < nc ? row[lane] * a0 : 0.0;
-        if (lane + 64 < nc) s = fma(row[lane + 64], a1, s);
-        s = wave_sum(s);
-        if (lane == 0) o[f] = s;
+    // Jacobian columns in groups of 128 (two per lane); one group for up to 42 atoms
+    for (int c0 = 0; c0 < nc; c0 += 128) {
+        const double a0 = c0 + lane < nc ? alpha[offs[b] + c0 + lane] : 0.0;
+        const double a1 = c0 + lane + 64 < nc ? alpha[offs[b] + c0 + lane + 64] : 0.0;
+        for (int f = 0; f < rep; ++f) {
+            const double *row = J + (size_t)f * ld + c0;
+            double s = c0 + lane < nc ? row[lane] * a0 : 0.0;
+            if (c0 + lane + 64 < nc) s = fma(row[lane + 64], a1, s);
+            s = wave_sum(s);
+            if (lane == 0) o[f] = c0 == 0 ? s : o[f] + s;
+        }
     }
 }
 
@@ -156,16 +158,20 @@ __global__ __launch_bounds__(NT) void backproj_kernel(const double *__restrict__
     const double *J = dX1 + (size_t)src * rep * ld;
     const double *g = G + (size_t)r * rep, *x = X1p + (size_t)r * rep;
     const double w = wsum[r];
-    double s0 = 0.0, s1 = 0.0;
-    for (int f = 0; f < rep; ++f) {
-        const double gv = g[f] - w * x[f];
-        const double *row = J + (size_t)f * ld;
-        if (lane < nc) s0 = fma(row[lane], gv, s0);
-        if (lane + 64 < nc) s1 = fma(row[lane + 64], gv, s1);
-    }
     double *o = Fat + (size_t)src * ld;
-    if (lane < nc) o[lane] = s0;
-    if (lane + 64 < nc) o[lane + 64] = s1;
+    // Jacobian columns in groups of 128 (two per lane); one group for up to 42 atoms
+    for (int c0 = 0; c0 < nc; c0 += 128) {
+        const int ca = c0 + lane, cb = c0 + lane + 64;
+        double s0 = 0.0, s1 = 0.0;
+        for (int f = 0; f < rep; ++f) {
+            const double gv = g[f] - w * x[f];
+            const double *row = J + (size_t)f * ld;
+            if (ca < nc) s0 = fma(row[ca], gv, s0);
+            if (cb < nc) s1 = fma(row[cb], gv, s1);
+        }
+        if (ca < nc) o[ca] = s0;
+        if (cb < nc) o[cb] = s1;
+    }
 }
 
 // E[a] = sum_i erow[a*ma+i]; F[offs[a] + c] = sum_i Fat[a*ma+i, c] (fixed order).
@@ -209,16 +215,20 @@ __global__ __launch_bounds__(NT) void backproj_plan_kernel(const double *__restr
     const double *J = dX1 + (size_t)src * rep * ld;
     const double *g = G + (size_t)r * rep, *x = X1p + (size_t)r * rep;
     const double w = wsum[r];
-    double s0 = 0.0, s1 = 0.0;
-    for (int f = 0; f < rep; ++f) {
-        const double gv = g[f] - w * x[f];
-        const double *row = J + (size_t)f * ld;
-        if (lane < nc) s0 = fma(row[lane], gv, s0);
-        if (lane + 64 < nc) s1 = fma(row[lane + 64], gv, s1);
-    }
     double *o = Fat + (size_t)src * ld;
-    if (lane < nc) o[lane] = s0;
-    if (lane + 64 < nc) o[lane + 64] = s1;
+    // Jacobian columns in groups of 128 (two per lane); one group for up to 42 atoms
+    for (int c0 = 0; c0 < nc; c0 += 128) {
+        const int ca = c0 + lane, cb = c0 + lane + 64;
+        double s0 = 0.0, s1 = 0.0;
+        for (int f = 0; f < rep; ++f) {
+            const double gv = g[f] - w * x[f];
+            const double *row = J + (size_t)f * ld;
+            if (ca < nc) s0 = fma(row[ca], gv, s0);
+            if (cb < nc) s1 = fma(row[cb], gv, s1);
+        }
+        if (ca < nc) o[ca] = s0;
+        if (cb < nc) o[cb] = s1;
+    }
 }
 
 // reduce_mol_kernel with atom counts from the force offsets.
@@ -257,8 +267,6 @@ int matvec_impl(const double *X1, const double *dX1, const double *X2, const dou
     KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
     KF_REQUIRE(nm1 >= 0 && nm2 >= 0 && ma1 >= 0 && ma2 >= 0 && rep >= 1, KF_ERR_ARG,
                "bad dimensions");
-    KF_REQUIRE(3 * ma1 <= MAXC, KF_ERR_BASIS, "matvec supports at most %d atoms per molecule",
-               MAXC / 3);
     if (nm1 == 0) return KF_OK;
     KF_REQUIRE(X1 && Q1 && N1 && (nm2 == 0 || (X2 && ad && Q2 && N2)), KF_ERR_ARG,
                "null pointer");
@@ -416,8 +424,6 @@ int plan_create(const int *Q1, const int *N1, int nm1, int ma1, const double *X2
     KF_REQUIRE(std::isfinite(sigma) && sigma > 0.0, KF_ERR_SIGMA, "sigma must be > 0");
     KF_REQUIRE(nm1 >= 1 && nm2 >= 0 && ma1 >= 1 && ma2 >= 0 && rep >= 1, KF_ERR_ARG,
                "bad dimensions");
-    KF_REQUIRE(3 * ma1 <= MAXC, KF_ERR_BASIS, "matvec supports at most %d atoms per molecule",
-               MAXC / 3);
     KF_REQUIRE(Q1 && N1 && (nm2 == 0 || (X2 && ad && aE && Q2 && N2)), KF_ERR_ARG,
                "null pointer");
     KF_HIP_TRY(hipGetDevice(&P->device));
@@ -570,7 +576,6 @@ extern "C" int kf_local_alpha_desc(const double *dX, const int *Q, const int *N,
     kf::EntryGuard kf_entry_guard_(kf::as_stream(stream));
     (void)Q;
     KF_REQUIRE(nm >= 0 && max_atoms >= 0 && rep >= 1, KF_ERR_ARG, "bad dimensions");
-    KF_REQUIRE(3 * max_atoms <= MAXC, KF_ERR_BASIS, "at most %d atoms per molecule", MAXC / 3);
     if (nm == 0 || max_atoms == 0) return KF_OK;
     KF_REQUIRE(dX && N && alpha && out, KF_ERR_ARG, "null pointer");
     hipStream_t st = as_stream(stream);

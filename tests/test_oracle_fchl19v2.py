@@ -91,3 +91,21 @@ def test_module_rejects_bad_types_and_sizes():
         m.generate(c, z, two_body_type="no_such_basis")
     with pytest.raises(ValueError):
         m.generate(c, z, three_body_type="cosine_split_r")  # nRs3_minus = 0
+
+
+def test_fchl19_is_the_v2_lognormal_odd_fourier_basis():
+    """FCHL19 equals FCHL19v2 with the log-normal two-body and odd-Fourier r_bar three-body
+    bases, values and Jacobians (the gradient path's first-harmonic convention included),
+    which lets the library run large molecules of FCHL19 on the FCHL19v2 kernel."""
+    from kernelforge_amd import synthetic
+
+    coords, zs, _ = synthetic.random_molecules(6, 3)
+    nd = dict(nFourier=2, nRs2=12, nRs3=9, eta2=0.41, eta3=2.1, rcut=5.5, acut=4.5,
+              two_body_decay=1.5, three_body_decay=0.7, three_body_weight=9.0)
+    for m in range(3):
+        c, z = coords[m], zs[m]
+        for kw in ({}, nd):
+            assert row_rel_err(v2.generate(c, z, **kw), oracle.fchl19(c, z, **kw)) < 1e-14
+            r1, g1 = oracle.fchl19(c, z, grad=True, **kw)
+            r2, g2 = v2.generate_and_gradients(c, z, **kw)
+            assert row_rel_err(r2, r1) < 1e-14 and row_rel_err(g2, g1) < 1e-13
