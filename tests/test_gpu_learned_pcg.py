@@ -140,21 +140,28 @@ def test_adam_step_matches_numpy():
 
 
 def test_training_reduces_unrolled_loss():
-    """Ablation: the trained network vs plain block-Jacobi (zero shift) and vs its own
-    initialisation, on the 10-step unrolled PCG loss of the systems it trains on; the
-    held-out system's numbers are printed (with 6 x 8 conformers the network does not
-    generalise; the workload-scale measurement is reported by bench.py)."""
+    """Ablation on a held-out system, as the 2,000-conformer workload is built: the network
+    trains on kernel systems of random 12-conformer subsets of a 40-conformer pool and is
+    then applied to the full 40-conformer system, which it never saw (its blocks couple
+    all 40 conformers).  The trained correction must lower the 10-step unrolled PCG loss
+    of that held-out system below plain block-Jacobi (zero shift) and below the network's
+    initialisation, and the training systems' mean loss as well.  (The objective is the
+    10-step residual; at lambda = 1e-2 it does not carry over to 100-iteration residuals,
+    which bench.py reports at the workload's size and lambda.)"""
     from kernelforge_amd import learned_pcg as L
 
     lam = 1e-2
-    C, Z, E, F = conformer_set(56, 7)
-    systems = []
-    for k in range(7):
-        sl = slice(8 * k, 8 * k + 8)
-        s = L.DeviceSystem(C[sl], Z, E[sl], F[sl])
+    C, Z, E, F = conformer_set(40, 7)
+    E = E - E.mean()
+    rng = np.random.default_rng(3)
+    train = []
+    for _ in range(8):
+        idx = np.sort(rng.choice(40, 12, replace=False))
+        s = L.DeviceSystem(C[idx], Z, E[idx], F[idx])
         s.build_rfp()
-        systems.append(s)
-    train, held = systems[:6], systems[6]
+        train.append(s)
+    held = L.DeviceSystem(C, Z, E, F)
+    held.build_rfp()
 
     def mean_loss(net_or_none, group):
         out = []
@@ -170,8 +177,8 @@ def test_training_reduces_unrolled_loss():
     hist = L.train(net, train, lam, epochs=15, steps=10, lr=0.003)
     trained_train, trained_held = mean_loss(net, train), mean_loss(net, [held])
     print(f"unrolled log-residual (10 steps), train: zero shift {zero_train:.4f}, "
-          f"initial net {init_train:.4f}, trained {trained_train:.4f}; held-out: zero shift "
-          f"{zero_held:.4f}, initial {init_held:.4f}, trained {trained_held:.4f}; "
-          f"epochs {np.round(hist, 4).tolist()}")
-    assert trained_train < init_train - 0.02
-    assert trained_train < zero_train - 0.02
+          f"initial net {init_train:.4f}, trained {trained_train:.4f}; held-out 40-conformer "
+          f"system: zero shift {zero_held:.4f}, initial {init_held:.4f}, trained "
+          f"{trained_held:.4f}; epochs {np.round(hist, 4).tolist()}")
+    assert trained_train < min(init_train, zero_train) - 0.05
+    assert trained_held < min(init_held, zero_held) - 0.05
