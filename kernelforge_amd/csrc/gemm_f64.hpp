@@ -35,38 +35,38 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int M, int N, int K, doub
     // B (K x N) tiles are 16 k x 64 columns; thread -> (k, 4 columns)
     const int bk = tid >> 4, bn = (tid & 15) * 4;
     double ra[4], rb[4];
+    // Loads are unconditional, of clamped (valid) elements, into registers; the padding
+    // is applied when the chunk is stored to LDS, by a select on a condition formed with
+    // non-short-circuit '&'.  A guarded load (or a '&&' condition, whose first comparison
+    // becomes a branch the load is sunk into) is waited for before the next one is
+    // issued, and a select right after the loads waits for them before the products of
+    // the current chunk.
+    const int am = min(m0 + lr, M - 1);
+    const bool okm = m0 + lr < M;
     auto load = [&](int k0) {
-        const int gm = m0 + lr;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int gk = k0 + lk + u;
-            ra[u] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
-        }
+        for (int u = 0; u < 4; ++u) ra[u] = A[(size_t)am * lda + min(k0 + lk + u, K - 1)];
         if (TRANS_B) {
-            const int gn = n0 + lr;
+            const size_t brow = (size_t)min(n0 + lr, N - 1) * ldb;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int gk = k0 + lk + u;
-                rb[u] = (gn < N && gk < K) ? B[(size_t)gn * ldb + gk] : 0.0;
-            }
+            for (int u = 0; u < 4; ++u) rb[u] = B[brow + min(k0 + lk + u, K - 1)];
         } else {
-            const int gk = k0 + bk;
+            const size_t brow = (size_t)min(k0 + bk, K - 1) * ldb;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int gn = n0 + bn + u;
-                rb[u] = (gn < N && gk < K) ? B[(size_t)gk * ldb + gn] : 0.0;
-            }
+            for (int u = 0; u < 4; ++u) rb[u] = B[brow + min(n0 + bn + u, N - 1)];
         }
     };
-    auto store = [&]() {
+    auto store = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) As[lk + u][lr] = ra[u];
+        for (int u = 0; u < 4; ++u) As[lk + u][lr] = (okm & (k0 + lk + u < K)) ? ra[u] : 0.0;
         if (TRANS_B) {
+            const bool okn = n0 + lr < N;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) Bs[lk + u][lr] = rb[u];
+            for (int u = 0; u < 4; ++u) Bs[lk + u][lr] = (okn & (k0 + lk + u < K)) ? rb[u] : 0.0;
         } else {
+            const bool okk = k0 + bk < K;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) Bs[bk][bn + u] = rb[u];
+            for (int u = 0; u < 4; ++u) Bs[bk][bn + u] = ((n0 + bn + u < N) & okk) ? rb[u] : 0.0;
         }
     };
     v4d_t c[2][2];
@@ -74,10 +74,10 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int M, int N, int K, doub
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) c[x][y] = (v4d_t){0.0, 0.0, 0.0, 0.0};
-    load(0);
+    if (K > 0) load(0);  // the clamped addresses need K >= 1 (M, N >= 1 by the launcher)
     for (int k0 = 0; k0 < K; k0 += KC) {
         __syncthreads();
-        store();
+        store(k0);
         __syncthreads();
         if (k0 + KC < K) load(k0 + KC);
 #pragma unroll

@@ -727,16 +727,24 @@ __device__ __forceinline__ void ff_pair(const DerivArgs &A, int a, int b, int na
     // rank-1 part: acc -= sum_m (w_m v_m) u_m^T, K = matched pairs
     {
         const double *V = slot + A.maxm, *U = V + (size_t)A.maxm * A.ldv;
+        // loads of clamped (valid) entries, unconditional, masked by selects on '&'
+        // conditions ('&&' turns into a branch around each load, which is then waited
+        // for before the next one); np >= 1 and nca, ncb >= 1 inside the loop
         for (int m0 = 0; m0 < np; m0 += 4) {
             const int m = m0 + kr;
             const bool ok = m < np;
-            const double w = ok ? -mc[m] * A.inv_s2 : 0.0;  // -e/s^4
+            const int mm = min(m, np - 1);
+            const double w = ok ? -mc[mm] * A.inv_s2 : 0.0;  // -e/s^4
             const int qq = 16 * qb + col;
-            const double bv = (ok && qq < ncb) ? U[(size_t)m * A.ldu + qq] : 0.0;
+            const double u = U[(size_t)mm * A.ldu + min(qq, ncb - 1)];
+            const double bv = (ok & (qq < ncb)) ? u : 0.0;
+            double vv[NP];
+#pragma unroll
+            for (int y = 0; y < NP; ++y) vv[y] = V[(size_t)mm * A.ldv + min(16 * y + col, nca - 1)];
 #pragma unroll
             for (int y = 0; y < NP; ++y) {
                 const int p = 16 * y + col;
-                const double av = (ok && p < nca) ? w * V[(size_t)m * A.ldv + p] : 0.0;
+                const double av = (ok & (p < nca)) ? w * vv[y] : 0.0;
                 acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[y], 0, 0, 0);
             }
         }

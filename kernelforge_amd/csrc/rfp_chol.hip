@@ -88,10 +88,13 @@ __device__ inline void load_tile(const T *__restrict__ a, const Tile &t, int rn,
         const int idx = threadIdx.x + NT * e;
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        // unconditional load of a clamped (valid) element, then select: a load under a
-        // branch is waited for before the next one is issued
+        // unconditional load of a clamped (valid) element, then select.  The condition is
+        // formed with non-short-circuit '&': with '&&' the compiler sinks the load into
+        // the branch of the first comparison, and every load of the unrolled loop is then
+        // waited for before the next one is issued.
         const T v = a[t.base + min(r, rn - 1) * t.rs + min(c, cn - 1) * t.cs];
-        dst[r * TP + c] = (r < rn && c < cn && (!LOWER_ONLY || r >= c)) ? v : T(0);
+        const bool keep = (r < rn) & (c < cn) & (!LOWER_ONLY | (r >= c));
+        dst[r * TP + c] = keep ? v : T(0);
     }
 }
 template <typename T, bool LOWER_ONLY>
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
                 const int y = wc * 32 + j * 16 + (lane & 15);
                 const int row = trans ? y : x, col = trans ? x : y;
                 const T c = a[tc.base + (size_t)min(row, rni - 1) * tc.rs + (size_t)min(col, rnk - 1) * tc.cs];
-                acc[i][j][v] = (row < rni && col < rnk && (!diagt || row >= col)) ? c : T(0);
+                acc[i][j][v] = ((row < rni) & (col < rnk) & (!diagt | (row >= col))) ? c : T(0);
             }
     // panel tiles: thread owns 16 elements of each (coalesced along the contiguous axis)
     T pi[16], pk[16];
@@ -394,9 +397,9 @@ __global__ __launch_bounds__(256) void syrk_range(T *a, Part P, int Jlo, int Jhi
             const int idx = threadIdx.x + 256 * e;
             int rr, cc;
             if (ti.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            Ai[rr * TP + cc] = (rr < rni && cc < w) ? pi[e] : T(0);
+            Ai[rr * TP + cc] = ((rr < rni) & (cc < w)) ? pi[e] : T(0);
             if (tk.rs == 1) { rr = idx % NB; cc = idx / NB; } else { cc = idx % NB; rr = idx / NB; }
-            Ak[rr * TP + cc] = (rr < rnk && cc < w) ? pk[e] : T(0);
+            Ak[rr * TP + cc] = ((rr < rnk) & (cc < w)) ? pk[e] : T(0);
         }
     };
     gload(Jlo);
@@ -492,7 +495,7 @@ __device__ __forceinline__ void syrk2_body(T *__restrict__ a, const Part &P, int
                     const int x = i * 16 + mfma16_row<T>(lane, v), y = j * 16 + (lane & 15);
                     const int row = trans ? y : x, col = trans ? x : y;
                     const T c = cb[min(row, rni - 1) * trs + min(col, rnk - 1) * tcs];
-                    acc[i][j][v] = (row < rni && col < rnk && (!diagt || row >= col)) ? c : T(0);
+                    acc[i][j][v] = ((row < rni) & (col < rnk) & (!diagt | (row >= col))) ? c : T(0);
                 }
     } else {
 #pragma unroll
@@ -706,37 +709,42 @@ __global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *
 }
 
 // The tile L(I, J) of a step, for block rows/columns of widths rn x cn, into registers
-// (thread t holds elements t + 256 e, coalesced along the contiguous axis; zero outside
-// the tile), so that its loads are in flight while the step forms its diagonal product.
+// (thread t holds elements t + 256 e, coalesced along the contiguous axis), so that its
+// loads are in flight while the step forms its diagonal product.  The loads are of clamped
+// (valid) elements and unconditional; the zero padding outside the tile is applied at the
+// LDS store, so nothing waits for them before the tile is needed.
 constexpr int SE_T = NB * NB / 256;
 template <typename T>
 __device__ __forceinline__ void step_tile_load(const T *a, const Tile &t, int rn, int cn,
-                                               double (&v)[SE_T]) {
+                                               T (&v)[SE_T]) {
 #pragma unroll
     for (int e = 0; e < SE_T; ++e) {
         const int idx = threadIdx.x + 256 * e;
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        const double x = (double)a[t.base + (size_t)min(r, rn - 1) * t.rs + (size_t)min(c, cn - 1) * t.cs];
-        v[e] = (r < rn && c < cn) ? x : 0.0;
+        v[e] = a[t.base + (size_t)min(r, rn - 1) * t.rs + (size_t)min(c, cn - 1) * t.cs];
     }
 }
-__device__ __forceinline__ void step_tile_store(const Tile &t, const double (&v)[SE_T], double *Lt) {
+template <typename T>
+__device__ __forceinline__ void step_tile_store(const Tile &t, int rn, int cn,
+                                                const T (&v)[SE_T], double *Lt) {
 #pragma unroll
     for (int e = 0; e < SE_T; ++e) {
         const int idx = threadIdx.x + 256 * e;
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        Lt[r * TP + c] = v[e];
+        Lt[r * TP + c] = ((r < rn) & (c < cn)) ? (double)v[e] : 0.0;  // '&', not '&&': see load_tile
     }
 }
 
 // forward step J: y_J = Linv_J b_J (stored to y by WG 0); b_I -= L_IJ y_J for I > J.
 // The products run over all 64 columns with the terms past the block width masked, so
 // every load of a product is issued at once.
+// A step is latency-bound (few work-groups per CU), so it may use the registers of two
+// waves per SIMD to keep all of its loads in flight at once.
 template <typename T>
-__global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const double *inv,
-                                                double *b, double *y, int n, int nrhs) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void fwd_step(const T *a, Part P, int J, const double *inv, double *b, double *y, int n, int nrhs) {
     __shared__ double yj[NB];
     __shared__ double Lt[NB * TP];
     const int I = J + blockIdx.x;  // blockIdx 0 -> the diagonal block itself
@@ -744,7 +752,7 @@ __global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const
     const double *Li = inv + (size_t)J * NB * NB;
     int r0 = 0, rn = 0;
     Tile t{};
-    double lt[SE_T];
+    T lt[SE_T];
     if (blockIdx.x > 0) {
         r0 = P.start(I);
         rn = P.width(I);
@@ -753,22 +761,28 @@ __global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
-        // y_J = Linv_J b_J: 4 threads per row
+        double bI = 0.0;  // this thread's row of b_I, loaded ahead of the products
+        if (blockIdx.x > 0) bI = b[(size_t)q * n + r0 + min((int)(threadIdx.x >> 2), rn - 1)];
+        // y_J = Linv_J b_J: 4 threads per row.  Every load is issued before the first
+        // product; Linv_J is zero past the block width, so the clamped b entries there
+        // add exact zeros.
         {
             const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
             const double *bq = b + (size_t)q * n + c0;
-            double s = 0.0;
+            double lv[NB / 4], bv[NB / 4];
 #pragma unroll
             for (int k = 0; k < NB / 4; ++k) {
-                const int c = qq + 4 * k;
-                const double lv = Li[r * NB + c], bv = bq[min(c, w - 1)];
-                s += c < w ? lv * bv : 0.0;
+                lv[k] = Li[r * NB + qq + 4 * k];
+                bv[k] = bq[min(qq + 4 * k, w - 1)];
             }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) s += lv[k] * bv[k];
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && r < NB) yj[r] = r < w ? s : 0.0;
         }
-        if (q == 0 && blockIdx.x > 0) step_tile_store(t, lt, Lt);
+        if (q == 0 && blockIdx.x > 0) step_tile_store(t, rn, w, lt, Lt);
         __syncthreads();
         if (blockIdx.x == 0) {
             if (threadIdx.x < w) y[(size_t)q * n + c0 + threadIdx.x] = yj[threadIdx.x];
@@ -782,15 +796,15 @@ __global__ __launch_bounds__(256) void fwd_step(const T *a, Part P, int J, const
             }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
-            if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] -= s;
+            if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] = bI - s;
         }
     }
 }
 
 // backward step J: x_J = Linv_J^T y_J (stored to x by WG 0); y_I -= L_JI^T x_J for I < J.
 template <typename T>
-__global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const double *inv,
-                                                double *y, double *x, int n, int nrhs) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void bwd_step(const T *a, Part P, int J, const double *inv, double *y, double *x, int n, int nrhs) {
     __shared__ double xj[NB];
     __shared__ double Lt[NB * TP];
     const int r0 = P.start(J), w = P.width(J);
@@ -798,7 +812,7 @@ __global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const
     const int I = (int)blockIdx.x - 1;  // blockIdx 0 -> the diagonal block
     int c0 = 0, cn = 0;
     Tile t{};
-    double lt[SE_T];
+    T lt[SE_T];
     if (blockIdx.x > 0) {
         c0 = P.start(I);
         cn = P.width(I);
@@ -807,22 +821,26 @@ __global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const
     }
     for (int q = 0; q < nrhs; ++q) {
         __syncthreads();
+        double yI = 0.0;  // this thread's column of y_I, loaded ahead of the products
+        if (blockIdx.x > 0) yI = y[(size_t)q * n + c0 + min((int)(threadIdx.x >> 2), cn - 1)];
         {
             // x_J[c] = sum_r Linv[r][c] y_J[r]
             const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
             const double *yq = y + (size_t)q * n + r0;
+            double lv[NB / 4], yv[NB / 4];
+#pragma unroll
+            for (int k = 0; k < NB / 4; ++k) {  // every load before the first product
+                lv[k] = Li[(qq + 4 * k) * NB + c];
+                yv[k] = yq[min(qq + 4 * k, w - 1)];
+            }
             double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < NB / 4; ++k) {
-                const int r = qq + 4 * k;
-                const double lv = Li[r * NB + c], yv = yq[min(r, w - 1)];
-                s += r < w ? lv * yv : 0.0;
-            }
+            for (int k = 0; k < NB / 4; ++k) s += lv[k] * yv[k];  // Linv_J zero past the width
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
             if (qq == 0 && c < NB) xj[c] = c < w ? s : 0.0;
         }
-        if (q == 0 && blockIdx.x > 0) step_tile_store(t, lt, Lt);
+        if (q == 0 && blockIdx.x > 0) step_tile_store(t, w, cn, lt, Lt);
         __syncthreads();
         if (blockIdx.x == 0) {
             if (threadIdx.x < w) x[(size_t)q * n + r0 + threadIdx.x] = xj[threadIdx.x];
@@ -836,7 +854,7 @@ __global__ __launch_bounds__(256) void bwd_step(const T *a, Part P, int J, const
             }
             s += __shfl_xor(s, 1);
             s += __shfl_xor(s, 2);
-            if (qq == 0 && c < cn) y[(size_t)q * n + c0 + c] -= s;
+            if (qq == 0 && c < cn) y[(size_t)q * n + c0 + c] = yI - s;
         }
     }
 }
