@@ -336,16 +336,17 @@ int kf_rfp_potrf(int precision, void *arf, int n, double l2, void *stream);
  * double b: the solve accumulates in FP64). */
 int kf_rfp_potrs(int precision, const void *arf, int n, double *b, int nrhs, void *stream);
 
-/* kf_rfp_potrs without allocation: the same solve, with its scratch (the 64x64 inverses
- * of the diagonal blocks and the intermediate L^-1 b) in a fixed 20 MiB device array
- * of the library instead of allocated workspace.  No counterpart in the reference: it is
- * for callers that cannot allocate device memory (code under graph capture, tight serving
- * loops that solve many right-hand sides with one factor).  Orders up to 32,768 with one
- * right-hand side fit (the scratch needs 8 * 4096 * nblk + 8 * n * nrhs bytes, nblk =
- * ceil((n/2)/64) + ceil((n - n/2)/64)); larger requests return KF_ERR_ARG.  The first
- * call on a device looks up the array's device address; no call allocates.  Calls on
- * one device share the array: calls that overlap on one device race (the library's
- * per-device serialisation of entry points orders calls on different streams). */
+/* kf_rfp_potrs without allocation: the same solve, with its scratch (the inverses of the
+ * diagonal blocks of groups of four 64-wide blocks, and the intermediate L^-1 b) in a fixed
+ * 48 MiB device array of the library instead of allocated workspace.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops that solve many right-hand sides with one factor).  Orders
+ * up to 38,912 with one right-hand side fit (the scratch needs 327,680 * ceil(nblk / 4) +
+ * 8 * n * nrhs bytes, nblk = ceil((n/2)/64) + ceil((n - n/2)/64)); larger requests return
+ * KF_ERR_ARG.  The first call on a device looks up the array's device address; no call
+ * allocates.  Calls on one device share the array: calls that overlap on one device race
+ * (the library's per-device serialisation of entry points orders calls on different
+ * streams). */
 int kf_rfp_potrs_noalloc(int precision, const void *arf, int n, double *b, int nrhs,
                          void *stream);
 

@@ -679,41 +679,144 @@ inline long range_tiles(const Part &P, int Kbeg, int Kend) {
 
 // ---------------------------------------------------------------------------
 // Triangular solves, FP64 accumulation.  b is (nrhs, n) row-major.
-// Diagonal blocks are inverted once per solve (all in parallel); each block step is
-// then one launch: every WG forms y_J = Linv_J b_J (64x64), WG 0 stores it, the
-// others apply it to their block.
+//
+// The 64-wide blocks are taken in groups of kGrp consecutive blocks.  Per solve, the
+// inverse X_g of each group's diagonal block of L (lower triangular, kGrp x kGrp tiles of
+// 64 x 64) is formed once: the diagonal tiles are the inverses of the 64 x 64 diagonal
+// blocks, and column j of tiles follows by block forward substitution,
+//   X(i, j) = -X(i, i) * sum_{k = j .. i-1} L(i, k) X(k, j).
+// A forward step over group g is then two launches: one forms y_g = X_g b_g (a work-group
+// per block of the group), the next applies it to every block row below the group, one
+// work-group each, b_I -= sum_j L(I, j) y_j; the backward steps mirror this with X_g^T and
+// the block columns left of the group.  The chain of dependent launches is 2 / kGrp of one
+// launch per 64-wide block, and X_g is read once per step.
+//
+// X storage: per group kGrpTiles row-major 64 x 64 FP64 tiles, tile (i, j) (j <= i) at
+// index i (i + 1) / 2 + j; entries outside the blocks' widths are zero.
 // ---------------------------------------------------------------------------
+constexpr int kGrp = 4;
+constexpr int kGrpTiles = kGrp * (kGrp + 1) / 2;
+constexpr int kTileElems = NB * NB;
+static_assert(kGrp * NB == 256, "a step maps one thread per (group block, row)");
+
+inline int n_groups(const Part &P) { return (P.nblk + kGrp - 1) / kGrp; }
+__host__ __device__ inline int gtile(int i, int j) { return i * (i + 1) / 2 + j; }
+__device__ inline double *xtile(double *X, int g, int i, int j) {
+    return X + ((size_t)g * kGrpTiles + gtile(i, j)) * kTileElems;
+}
+__device__ inline const double *xtile(const double *X, int g, int i, int j) {
+    return X + ((size_t)g * kGrpTiles + gtile(i, j)) * kTileElems;
+}
+
+// Diagonal tiles of every group's X: the inverses of the 64 x 64 diagonal blocks.
 template <typename T>
-__global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *inv) {
+__global__ __launch_bounds__(256) void inv_diag_all(const T *a, Part P, double *X) {
     __shared__ double L[NB * TP];
     __shared__ double Li[NB * TP];
     const int J = blockIdx.x;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < kTileElems / 256; ++e) {  // clamped loads, masked at the LDS store
+        const int idx = threadIdx.x + 256 * e;
         int r, c;
         if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        double v = 0.0;
-        if (r < w && c < w && r >= c) v = (double)a[t.base + (size_t)r * t.rs + (size_t)c * t.cs];
-        L[r * TP + c] = v;
+        const T v = a[t.base + (size_t)min(r, w - 1) * t.rs + (size_t)min(c, w - 1) * t.cs];
+        L[r * TP + c] = ((r < w) & (c < w) & (r >= c)) ? (double)v : 0.0;
         Li[r * TP + c] = 0.0;
     }
     __shared__ double dgi[NB];
     __syncthreads();
     tri_inverse<double, double>(L, Li, dgi, w);
-    double *dst = inv + (size_t)J * NB * NB;
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+    double *dst = xtile(X, J / kGrp, J % kGrp, J % kGrp);
+    for (int idx = threadIdx.x; idx < kTileElems; idx += blockDim.x) {
         const int r = idx / NB, c = idx % NB;
         dst[idx] = (r < w && c < w) ? Li[r * TP + c] : 0.0;
     }
 }
 
-// The tile L(I, J) of a step, for block rows/columns of widths rn x cn, into registers
-// (thread t holds elements t + 256 e, coalesced along the contiguous axis), so that its
-// loads are in flight while the step forms its diagonal product.  The loads are of clamped
-// (valid) elements and unconditional; the zero padding outside the tile is applied at the
-// LDS store, so nothing waits for them before the tile is needed.
-constexpr int SE_T = NB * NB / 256;
+// 64 x 64 FP64 product of two LDS tiles (pitch TP), 4 x 4 outputs per thread:
+// acc[u][v] += sum_k A[(4 ty + u) * TP + k] * B[k * TP + 4 tx + v].
+__device__ __forceinline__ void tile_mm(const double *A, const double *B, double (&acc)[4][4]) {
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+#pragma unroll 4
+    for (int k = 0; k < NB; ++k) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[u] = A[(4 * ty + u) * TP + k];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bv[v] = B[k * TP + 4 * tx + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    }
+}
+
+// A row-major 64 x 64 FP64 tile from global memory into LDS (pitch TP).
+__device__ __forceinline__ void xtile_to_lds(const double *src, double *dst) {
+#pragma unroll
+    for (int e = 0; e < kTileElems / 256; ++e) {
+        const int idx = threadIdx.x + 256 * e;
+        dst[(idx / NB) * TP + idx % NB] = src[idx];
+    }
+}
+
+// Off-diagonal tiles of X: one work-group per (group, tile column j < kGrp - 1), walking
+// the rows i = j+1 .. m-1 of its column in order (each needs the tiles above it).
+template <typename T>
+__global__ __launch_bounds__(256) void inv_group_offdiag(const T *a, Part P, double *X) {
+    __shared__ double As[NB * TP];
+    __shared__ double Bs[NB * TP];
+    const int g = blockIdx.x / (kGrp - 1), j = blockIdx.x % (kGrp - 1);
+    const int J0 = g * kGrp, m = min(kGrp, P.nblk - J0);
+    if (j + 1 >= m) return;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    for (int i = j + 1; i < m; ++i) {
+        double acc[4][4] = {};
+        const int ri = P.start(J0 + i), wi = P.width(J0 + i);
+        for (int k = j; k < i; ++k) {
+            __syncthreads();  // the previous product is done with As and Bs
+            const Tile t = tile_at(P, ri, P.start(J0 + k));
+            const int wk = P.width(J0 + k);
+#pragma unroll
+            for (int e = 0; e < kTileElems / 256; ++e) {  // L(i, k), zero outside the blocks
+                const int idx = threadIdx.x + 256 * e;
+                int r, c;
+                if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
+                const T v = a[t.base + (size_t)min(r, wi - 1) * t.rs + (size_t)min(c, wk - 1) * t.cs];
+                As[r * TP + c] = ((r < wi) & (c < wk)) ? (double)v : 0.0;
+            }
+            // X(k, j): written by inv_diag_all (k == j) or by this work-group's earlier rows,
+            // whose stores the barriers order before these loads
+            xtile_to_lds(xtile(X, g, k, j), Bs);
+            __syncthreads();
+            tile_mm(As, Bs, acc);
+        }
+        __syncthreads();
+        xtile_to_lds(xtile(X, g, i, i), As);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Bs[(4 * ty + u) * TP + 4 * tx + v] = acc[u][v];
+        __syncthreads();
+        double out[4][4] = {};
+        tile_mm(As, Bs, out);
+        double *dst = xtile(X, g, i, j);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dst[(4 * ty + u) * NB + 4 * tx + v] = -out[u][v];
+        __syncthreads();  // the stores of X(i, j) before the next row reads them
+    }
+}
+
+// Update tiles of a step in registers: thread t holds elements t + 256 e of each tile,
+// coalesced along its contiguous axis: in the column-major region (rs == 1) row t % 64 of
+// columns t / 64 + 4 e; in the row-major region column t % 64 of rows t / 64 + 4 e.  The
+// loads are of clamped (valid) elements and unconditional; the padding is applied where
+// the values are used (a select right after a load would wait for it there).
+constexpr int SE_T = kTileElems / 256;
 template <typename T>
 __device__ __forceinline__ void step_tile_load(const T *a, const Tile &t, int rn, int cn,
                                                T (&v)[SE_T]) {
@@ -725,138 +828,197 @@ __device__ __forceinline__ void step_tile_load(const T *a, const Tile &t, int rn
         v[e] = a[t.base + (size_t)min(r, rn - 1) * t.rs + (size_t)min(c, cn - 1) * t.cs];
     }
 }
-template <typename T>
-__device__ __forceinline__ void step_tile_store(const Tile &t, int rn, int cn,
-                                                const T (&v)[SE_T], double *Lt) {
+
+// Sum over the 64 lanes of a wave of 16 values per lane: on return, the lanes with
+// (lane & 3) == 0 hold the total of value index e = lane >> 2 (the others partial sums).
+// Four halving exchanges (xor 32, 16, 8, 4) each keep half of the remaining indices, then
+// two plain exchanges finish the sums: 17 exchanges instead of 16 x 6.
+__device__ __forceinline__ double wave_sum16(double (&v)[16]) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int e = 0; e < SE_T; ++e) {
-        const int idx = threadIdx.x + 256 * e;
-        int r, c;
-        if (t.rs == 1) { r = idx % NB; c = idx / NB; } else { c = idx % NB; r = idx / NB; }
-        Lt[r * TP + c] = ((r < rn) & (c < cn)) ? (double)v[e] : 0.0;  // '&', not '&&': see load_tile
+    for (int h = 8, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool up = (lane & bit) != 0;  // keep indices [h, 2h) of the current half
+#pragma unroll
+        for (int e = 0; e < h; ++e) {
+            const double keep = up ? v[e + h] : v[e], send = up ? v[e] : v[e + h];
+            v[e] = keep + __shfl_xor(send, bit);
+        }
+    }
+    double s = v[0];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    return s;
+}
+
+// Forward step over group g, right-hand side blockIdx.y, as two launches.
+// fwd_group_y: work-group i forms block i of y_g = X_g b_g, y_i = sum_{j <= i} X(i, j) b_j
+// (4 threads per row), and stores it to y.
+__global__ __launch_bounds__(256) void fwd_group_y(Part P, int g, const double *X,
+                                                   const double *b, double *y, int n) {
+    __shared__ double bg[kGrp][NB];
+    const int J0 = g * kGrp, i = blockIdx.x, tid = threadIdx.x;
+    const double *bq = b + (size_t)blockIdx.y * n;
+    {
+        const int j = tid >> 6, c = tid & 63;
+        const int jc = min(j, i), wj = P.width(J0 + jc);  // blocks j <= i < m exist
+        const double v = bq[P.start(J0 + jc) + min(c, wj - 1)];
+        bg[j][c] = ((j <= i) & (c < wj)) ? v : 0.0;
+    }
+    __syncthreads();
+    const int r = tid >> 2, qq = tid & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {  // blocks j > i of bg are zero: tile (i, i) stands in
+        const double *xt = xtile(X, g, i, min(j, i)) + r * NB + qq;
+#pragma unroll
+        for (int k = 0; k < NB / 4; ++k) s += xt[4 * k] * bg[j][qq + 4 * k];
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (qq == 0 && r < P.width(J0 + i)) y[(size_t)blockIdx.y * n + P.start(J0 + i) + r] = s;
+}
+
+// fwd_group_upd: b_I -= sum_j L(I, J0 + j) y_j for the block row I = J0 + m + blockIdx.x
+// below the group, from the update tiles in registers (loaded while y_g is read).
+template <typename T>
+__global__ __launch_bounds__(256) void fwd_group_upd(const T *a, Part P, int g, const double *y,
+                                                     double *b, int n) {
+    __shared__ double yg[kGrp][NB];
+    __shared__ double red[4][NB];
+    const int J0 = g * kGrp, m = min(kGrp, P.nblk - J0);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int I = J0 + m + (int)blockIdx.x;
+    const int r0 = P.start(I), rn = P.width(I);
+    const bool cm = tile_at(P, r0, P.start(J0)).rs == 1;  // block row I lies in one region
+    T lt[kGrp][SE_T];
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {  // clamped to the group's last block: valid tiles
+        const int jc = min(j, m - 1);
+        step_tile_load(a, tile_at(P, r0, P.start(J0 + jc)), rn, P.width(J0 + jc), lt[j]);
+    }
+    double *bq = b + (size_t)blockIdx.y * n;
+    const double bI = bq[r0 + min(lane, rn - 1)];
+    {
+        const int j = tid >> 6, c = tid & 63;
+        const int jc = min(j, m - 1), wj = P.width(J0 + jc);
+        const double v = y[(size_t)blockIdx.y * n + P.start(J0 + jc) + min(c, wj - 1)];
+        yg[j][c] = ((j < m) & (c < wj)) ? v : 0.0;  // zero past the group and the widths
+    }
+    __syncthreads();
+    if (cm) {  // thread: row lane, columns wv + 4e -> partial row sums, then across waves
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < kGrp; ++j) {
+            const int wj = P.width(J0 + min(j, m - 1));
+#pragma unroll
+            for (int e = 0; e < SE_T; ++e) {
+                const int c = wv + 4 * e;
+                s += (c < wj ? (double)lt[j][e] : 0.0) * yg[j][c];
+            }
+        }
+        red[wv][lane] = s;
+    } else {  // thread: column lane, rows wv + 4e -> per-row partials, summed over lanes
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = 0.0;
+#pragma unroll
+        for (int j = 0; j < kGrp; ++j) {
+            const double yv = yg[j][lane];  // zero past the block width
+#pragma unroll
+            for (int e = 0; e < SE_T; ++e) v[e] += (double)lt[j][e] * yv;
+        }
+        const double tot = wave_sum16(v);  // row wv + 4 (lane >> 2) on lanes 4k
+        if ((lane & 3) == 0) red[0][wv + 4 * (lane >> 2)] = tot;
+    }
+    __syncthreads();
+    if (tid < NB && tid < rn) {
+        const double tot = cm ? red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid] : red[0][tid];
+        bq[r0 + tid] = bI - tot;
     }
 }
 
-// forward step J: y_J = Linv_J b_J (stored to y by WG 0); b_I -= L_IJ y_J for I > J.
-// The products run over all 64 columns with the terms past the block width masked, so
-// every load of a product is issued at once.
-// A step is latency-bound (few work-groups per CU), so it may use the registers of two
-// waves per SIMD to keep all of its loads in flight at once.
-template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void fwd_step(const T *a, Part P, int J, const double *inv, double *b, double *y, int n, int nrhs) {
-    __shared__ double yj[NB];
-    __shared__ double Lt[NB * TP];
-    const int I = J + blockIdx.x;  // blockIdx 0 -> the diagonal block itself
-    const int c0 = P.start(J), w = P.width(J);
-    const double *Li = inv + (size_t)J * NB * NB;
-    int r0 = 0, rn = 0;
-    Tile t{};
-    T lt[SE_T];
-    if (blockIdx.x > 0) {
-        r0 = P.start(I);
-        rn = P.width(I);
-        t = tile_at(P, r0, c0);
-        step_tile_load(a, t, rn, w, lt);
+// Backward step over group g, right-hand side blockIdx.y, as two launches.
+// bwd_group_x: work-group j forms block j of x_g = X_g^T y_g,
+// x_j = sum_{i >= j} X(i, j)^T y_i (4 threads per column), and stores it to x.
+__global__ __launch_bounds__(256) void bwd_group_x(Part P, int g, const double *X,
+                                                   const double *y, double *x, int n) {
+    __shared__ double yg[kGrp][NB];
+    const int J0 = g * kGrp, m = min(kGrp, P.nblk - J0), j = blockIdx.x, tid = threadIdx.x;
+    const double *yq = y + (size_t)blockIdx.y * n;
+    {
+        const int i = tid >> 6, r = tid & 63;
+        const int ic = min(max(i, j), m - 1), wi = P.width(J0 + ic);  // blocks j <= i < m
+        const double v = yq[P.start(J0 + ic) + min(r, wi - 1)];
+        yg[i][r] = ((i >= j) & (i < m) & (r < wi)) ? v : 0.0;
     }
-    for (int q = 0; q < nrhs; ++q) {
-        __syncthreads();
-        double bI = 0.0;  // this thread's row of b_I, loaded ahead of the products
-        if (blockIdx.x > 0) bI = b[(size_t)q * n + r0 + min((int)(threadIdx.x >> 2), rn - 1)];
-        // y_J = Linv_J b_J: 4 threads per row.  Every load is issued before the first
-        // product; Linv_J is zero past the block width, so the clamped b entries there
-        // add exact zeros.
-        {
-            const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
-            const double *bq = b + (size_t)q * n + c0;
-            double lv[NB / 4], bv[NB / 4];
+    __syncthreads();
+    const int c = tid >> 2, qq = tid & 3;
+    double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < NB / 4; ++k) {
-                lv[k] = Li[r * NB + qq + 4 * k];
-                bv[k] = bq[min(qq + 4 * k, w - 1)];
-            }
-            double s = 0.0;
+    for (int i = 0; i < kGrp; ++i) {  // blocks i < j or i >= m of yg are zero: a valid tile
+        const double *xt = xtile(X, g, min(max(i, j), m - 1), j) + qq * NB + c;  // stands in
 #pragma unroll
-            for (int k = 0; k < NB / 4; ++k) s += lv[k] * bv[k];
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            if (qq == 0 && r < NB) yj[r] = r < w ? s : 0.0;
-        }
-        if (q == 0 && blockIdx.x > 0) step_tile_store(t, rn, w, lt, Lt);
-        __syncthreads();
-        if (blockIdx.x == 0) {
-            if (threadIdx.x < w) y[(size_t)q * n + c0 + threadIdx.x] = yj[threadIdx.x];
-        } else {
-            const int r = threadIdx.x >> 2, qq = threadIdx.x & 3;
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NB / 4; ++k) {
-                const int c = qq + 4 * k;
-                s += Lt[r * TP + c] * yj[c];  // zero past the width on both sides
-            }
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            if (qq == 0 && r < rn) b[(size_t)q * n + r0 + r] = bI - s;
-        }
+        for (int k = 0; k < NB / 4; ++k) s += xt[4 * k * NB] * yg[i][qq + 4 * k];
     }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (qq == 0 && c < P.width(J0 + j)) x[(size_t)blockIdx.y * n + P.start(J0 + j) + c] = s;
 }
 
-// backward step J: x_J = Linv_J^T y_J (stored to x by WG 0); y_I -= L_JI^T x_J for I < J.
+// bwd_group_upd: y_I -= sum_j L(J0 + j, I)^T x_j for the block column I = blockIdx.x left of
+// the group, from the update tiles in registers (loaded while x_g is read).
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void bwd_step(const T *a, Part P, int J, const double *inv, double *y, double *x, int n, int nrhs) {
-    __shared__ double xj[NB];
-    __shared__ double Lt[NB * TP];
-    const int r0 = P.start(J), w = P.width(J);
-    const double *Li = inv + (size_t)J * NB * NB;
-    const int I = (int)blockIdx.x - 1;  // blockIdx 0 -> the diagonal block
-    int c0 = 0, cn = 0;
-    Tile t{};
-    T lt[SE_T];
-    if (blockIdx.x > 0) {
-        c0 = P.start(I);
-        cn = P.width(I);
-        t = tile_at(P, r0, c0);
-        step_tile_load(a, t, w, cn, lt);
+__global__ __launch_bounds__(256) void bwd_group_upd(const T *a, Part P, int g, const double *x,
+                                                     double *y, int n) {
+    __shared__ double xg[kGrp][NB];
+    __shared__ double red[4][NB];
+    __shared__ double colA[NB];
+    const int J0 = g * kGrp, m = min(kGrp, P.nblk - J0);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int I = blockIdx.x;
+    const int c0 = P.start(I), cn = P.width(I);
+    T lt[kGrp][SE_T];
+    bool cmj[kGrp];  // tile L(J0 + j, I) in the column-major region
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {
+        const int jc = min(j, m - 1);
+        const Tile t = tile_at(P, P.start(J0 + jc), c0);
+        cmj[j] = t.rs == 1;
+        step_tile_load(a, t, P.width(J0 + jc), cn, lt[j]);
     }
-    for (int q = 0; q < nrhs; ++q) {
-        __syncthreads();
-        double yI = 0.0;  // this thread's column of y_I, loaded ahead of the products
-        if (blockIdx.x > 0) yI = y[(size_t)q * n + c0 + min((int)(threadIdx.x >> 2), cn - 1)];
-        {
-            // x_J[c] = sum_r Linv[r][c] y_J[r]
-            const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
-            const double *yq = y + (size_t)q * n + r0;
-            double lv[NB / 4], yv[NB / 4];
+    double *yq = y + (size_t)blockIdx.y * n;
+    const double yI = yq[c0 + min(lane, cn - 1)];
+    {
+        const int j = tid >> 6, r = tid & 63;
+        const int jc = min(j, m - 1), wj = P.width(J0 + jc);
+        const double v = x[(size_t)blockIdx.y * n + P.start(J0 + jc) + min(r, wj - 1)];
+        xg[j][r] = ((j < m) & (r < wj)) ? v : 0.0;  // zero past the group and the widths
+    }
+    __syncthreads();
+    double pB = 0.0, v[16];
 #pragma unroll
-            for (int k = 0; k < NB / 4; ++k) {  // every load before the first product
-                lv[k] = Li[(qq + 4 * k) * NB + c];
-                yv[k] = yq[min(qq + 4 * k, w - 1)];
+    for (int e = 0; e < 16; ++e) v[e] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {
+        const int wj = P.width(J0 + min(j, m - 1));
+        if (cmj[j]) {  // thread: row lane, columns wv + 4e -> per-column partials
+            const double xv = xg[j][lane];  // zero past the block width
+#pragma unroll
+            for (int e = 0; e < SE_T; ++e) v[e] += (double)lt[j][e] * xv;
+        } else {  // thread: column lane, rows wv + 4e -> partial of column lane
+#pragma unroll
+            for (int e = 0; e < SE_T; ++e) {
+                const int r = wv + 4 * e;
+                pB += (r < wj ? (double)lt[j][e] : 0.0) * xg[j][r];
             }
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NB / 4; ++k) s += lv[k] * yv[k];  // Linv_J zero past the width
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            if (qq == 0 && c < NB) xj[c] = c < w ? s : 0.0;
-        }
-        if (q == 0 && blockIdx.x > 0) step_tile_store(t, w, cn, lt, Lt);
-        __syncthreads();
-        if (blockIdx.x == 0) {
-            if (threadIdx.x < w) x[(size_t)q * n + r0 + threadIdx.x] = xj[threadIdx.x];
-        } else {
-            const int c = threadIdx.x >> 2, qq = threadIdx.x & 3;
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NB / 4; ++k) {
-                const int r = qq + 4 * k;
-                s += Lt[r * TP + c] * xj[r];  // zero past the width on both sides
-            }
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            if (qq == 0 && c < cn) y[(size_t)q * n + c0 + c] = yI - s;
         }
     }
+    const double totA = wave_sum16(v);  // column wv + 4 (lane >> 2) on lanes 4k
+    if ((lane & 3) == 0) colA[wv + 4 * (lane >> 2)] = totA;
+    red[wv][lane] = pB;
+    __syncthreads();
+    if (tid < NB && tid < cn)
+        yq[c0 + tid] = yI - (colA[tid] + red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
 }
 
 // Outer panels of 64-wide blocks: 8 (a 512-deep trailing update) while more than
@@ -939,30 +1101,47 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     return h;
 }
 
-// Scratch of a solve: the diagonal-block inverses, then the intermediate y = L^-1 b.
-// Every byte a solve reads from it is first written by the same solve (inv_diag_all
-// writes whole 64x64 blocks, the forward steps every row of y), so it needs no clearing.
-inline size_t potrs_inv_bytes(const Part &P) { return sizeof(double) * (size_t)P.nblk * NB * NB; }
+// Scratch of a solve: the group inverses X, then the intermediate y = L^-1 b.  Every byte
+// a solve reads from it is first written by the same solve (inv_diag_all and
+// inv_group_offdiag write the whole tiles a group uses, the forward steps every row of y),
+// so it needs no clearing.
+inline size_t potrs_x_bytes(const Part &P) {
+    return sizeof(double) * (size_t)n_groups(P) * kGrpTiles * kTileElems;
+}
 inline size_t potrs_scratch_bytes(const Part &P, int n, int nrhs) {
-    return potrs_inv_bytes(P) + sizeof(double) * (size_t)n * nrhs;
+    return potrs_x_bytes(P) + sizeof(double) * (size_t)n * nrhs;
 }
 
 template <typename T>
 int potrs_run(const T *a, const Part &P, double *b, int nrhs, char *scratch, hipStream_t stream,
               bool timed) {
-    const int n = P.n;
-    double *inv = reinterpret_cast<double *>(scratch);
-    double *y = reinterpret_cast<double *>(scratch + potrs_inv_bytes(P));
+    const int n = P.n, ng = n_groups(P);
+    double *X = reinterpret_cast<double *>(scratch);
+    double *y = reinterpret_cast<double *>(scratch + potrs_x_bytes(P));
     // the kernel timer creates HIP events when enabled, so the allocation-free entry
     // point runs untimed
     ScopedTimer timer(timed ? TM_POTRS : -1, stream);
-    hipLaunchKernelGGL(inv_diag_all<T>, dim3(P.nblk), dim3(256), 0, stream, a, P, inv);
-    for (int J = 0; J < P.nblk; ++J)
-        hipLaunchKernelGGL(fwd_step<T>, dim3(P.nblk - J), dim3(256), 0, stream, a, P, J, inv, b,
-                           y, n, nrhs);
-    for (int J = P.nblk - 1; J >= 0; --J)
-        hipLaunchKernelGGL(bwd_step<T>, dim3(J + 1), dim3(256), 0, stream, a, P, J, inv, y, b, n,
-                           nrhs);
+    hipLaunchKernelGGL(inv_diag_all<T>, dim3(P.nblk), dim3(256), 0, stream, a, P, X);
+    hipLaunchKernelGGL(inv_group_offdiag<T>, dim3(ng * (kGrp - 1)), dim3(256), 0, stream, a, P, X);
+    // right-hand sides along the grid's y dimension, at most 65,535 per launch
+    for (int q0 = 0; q0 < nrhs; q0 += 65535) {
+        const int nq = min(nrhs - q0, 65535);
+        double *bq = b + (size_t)q0 * n, *yq = y + (size_t)q0 * n;
+        for (int g = 0; g < ng; ++g) {
+            const int m = min(kGrp, P.nblk - g * kGrp), below = P.nblk - g * kGrp - m;
+            hipLaunchKernelGGL(fwd_group_y, dim3(m, nq), dim3(256), 0, stream, P, g, X, bq, yq, n);
+            if (below > 0)
+                hipLaunchKernelGGL(fwd_group_upd<T>, dim3(below, nq), dim3(256), 0, stream, a, P,
+                                   g, yq, bq, n);
+        }
+        for (int g = ng - 1; g >= 0; --g) {
+            const int m = min(kGrp, P.nblk - g * kGrp);
+            hipLaunchKernelGGL(bwd_group_x, dim3(m, nq), dim3(256), 0, stream, P, g, X, yq, bq, n);
+            if (g > 0)
+                hipLaunchKernelGGL(bwd_group_upd<T>, dim3(g * kGrp, nq), dim3(256), 0, stream, a,
+                                   P, g, bq, yq, n);
+        }
+    }
     KF_HIP_TRY(hipGetLastError());
     return KF_OK;
 }
@@ -979,7 +1158,7 @@ int potrs_impl(const T *a, int n, double *b, int nrhs, hipStream_t stream) {
 // ---------------------------------------------------------------------------
 // Allocation-free solve: the same kernels with their scratch in a fixed device array.
 // ---------------------------------------------------------------------------
-constexpr size_t kNoAllocScratchBytes = size_t(20) << 20;  // 20 MiB per device
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;  // 48 MiB per device
 static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDev = 16;

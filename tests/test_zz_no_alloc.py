@@ -52,11 +52,12 @@ def test_noalloc_matches_potrs_two_calls(prec):
 
 
 def _scratch_need(n, nrhs):
-    """Bytes of scratch a solve of order n needs: 64 x 64 FP64 inverse per diagonal block
-    (blocks of 64 inside [0, n/2) and inside [n/2, n)) plus the n x nrhs intermediate."""
+    """Bytes of scratch a solve of order n needs: the inverses of the diagonal blocks of
+    groups of four 64-wide blocks (ten 64 x 64 FP64 tiles per group; blocks of 64 inside
+    [0, n/2) and inside [n/2, n)) plus the n x nrhs intermediate."""
     k = n // 2
     nblk = -(-k // 64) + -(-(n - k) // 64)
-    return 8 * nblk * 64 * 64 + 8 * n * nrhs
+    return 8 * -(-nblk // 4) * 10 * 64 * 64 + 8 * n * nrhs
 
 
 @pytest.mark.cpu_only
@@ -69,8 +70,8 @@ def test_noalloc_rejects_orders_beyond_its_scratch():
     if L.kf_device_name(None, 0) == capi.KF_OK:
         pytest.skip("placeholder device pointers: CPU only")
     noalloc = _noalloc(L)
-    limit = 20 << 20
-    for n, nrhs in ((60000, 1), (20000, 200)):
+    limit = 48 << 20
+    for n, nrhs in ((60000, 1), (20000, 400)):
         assert _scratch_need(n, nrhs) > 1.4 * limit
         assert noalloc(capi.KF_PREC_F64, 4096, n, 4096, nrhs, None) == capi.KF_ERR_ARG
         assert b"scratch" in L.kf_last_error()
