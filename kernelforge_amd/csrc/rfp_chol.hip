@@ -1052,17 +1052,19 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
     hipStream_t s2 = aux_stream();
     hipEvent_t ev_start = aux_event(0), ev_f = aux_event(1), ev_n = aux_event(2);
     KF_REQUIRE(s2 && ev_start && ev_f && ev_n, KF_ERR_HIP, "aux stream/event creation failed");
+    // Inside an outer panel the updates are left-looking: block column J receives the
+    // contributions of blocks J0 .. J-1 in one deep update just before it is factored, so
+    // the panel chain runs one update launch per block with the C tiles read and written
+    // once, instead of a rank-64 update of every remaining panel column after each block.
     auto factor = [&](int J0, int J1) {
         for (int J = J0; J < J1; ++J) {
+            if (J > J0)
+                hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)range_tiles(P, J, J + 1)),
+                                   dim3(256), 0, s2, a, P, J0, J, J, J + 1, info);
             hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s2, a, P, J, inv, info);
             const int m = P.nblk - J - 1;
             if (m > 0)
                 hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, s2, a, P, J, inv, info);
-            if (J + 1 < J1) {  // remaining columns of this outer panel
-                const long nt = range_tiles(P, J + 1, J1);
-                hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)nt), dim3(256), 0, s2, a, P, J,
-                                   J + 1, J + 1, J1, info);
-            }
         }
     };
     auto next = [&](int J) {  // end of the outer panel that starts at block J
