@@ -10,7 +10,8 @@
 //  * Atoms are packed by (label, molecule) once (local_plan.hpp), so the rows of a
 //    molecule block for one label are one contiguous range.
 //  * The OUTPUT is tiled: one workgroup owns a 64x64 block of molecule pairs (upper
-//    triangle of blocks for the symmetric forms), accumulated in LDS.  For every label
+//    triangle of blocks for the symmetric forms), accumulated in a device-memory slot of
+//    the work-group (three work-groups per CU fit in LDS and registers).  For every label
 //    it runs the rows(A-block) x rows(B-block) Gram product as 64-row sub-tiles on FP64
 //    MFMA (v_mfma_f64_16x16x4): one row tile against two column tiles at a time; wave
 //    w owns columns 16w..16w+15 and skips empty 16-row/16-column fragments (95% of
@@ -19,10 +20,11 @@
 //    operands stream from global memory straight into registers.
 //  * Epilogue fused per tile, in registers: squared norms (FP64), exp, a segmented
 //    scan along each wave's 16 columns (DPP row shifts) that sums the columns of one
-//    molecule, then the row molecules' sums into the LDS block.  Deterministic, no
-//    atomics; each K entry is written exactly once, straight into RFP (UPLO='U') or
-//    dense layout.
+//    molecule, then the row molecules' sums into the block's accumulators.  Deterministic
+//    (no atomics on values); each K entry is written exactly once, straight into RFP
+//    (UPLO='U') or dense layout.
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -39,10 +41,41 @@ constexpr int KA = 32;       // row-operand chunk staged in LDS (descriptor colu
 constexpr int KS = 16;       // column-operand sub-chunk held in registers
 constexpr int LDA = KA + 2;  // LDS pitch of the staged rows (conflict-free 16-byte reads)
 constexpr int PP = TS + 4;   // LDS pitch of the segment sums (<= 64 molecules + 3 window cuts)
-constexpr int AP = MB + 1;   // LDS pitch of the block accumulators
+constexpr int AP = MB;       // row pitch of the block accumulators
 constexpr int NT = 256;      // threads per workgroup
 
 enum Mode { SYMM_RFP = 0, SYMM_DENSE = 1, RECT = 2 };
+
+// Block accumulators in device memory.  The 64 x 64 FP64 sums of a work-group's output
+// block (32 KB) live in one of the slots of a fixed device array instead of LDS, which
+// brings the work-group to 40 KB of LDS so that three fit on a CU (with 168 registers):
+// 207 ms instead of 222 ms at 20,000 molecules with the accumulators in LDS at two per CU.
+// The slots are split by XCD (read from the hardware's XCC id): a work-group claims a free
+// slot of its own XCD at its start (compare-and-swap on a busy flag) and frees it at its
+// end, so a slot's cache lines only ever live in one XCD's L2 -- a slot handed between
+// XCDs could be overwritten later by the other L2's write-back of the previous owner's
+// lines.  Fewer work-groups than slots are resident on an XCD (the host checks), so the
+// probe ends; the flags are cleared by a small kernel before every launch.  Only the owner
+// thread of an entry touches it between two barriers, as in LDS; launches on one device do
+// not overlap (the entry points order them).
+constexpr int kXcc = 8, kSlotsPerXcc = 128, kSlots = kXcc * kSlotsPerXcc;
+__device__ double g_acc_slots[(size_t)kSlots * 64 * 64];
+__device__ int g_slot_busy[kSlots];
+
+__device__ __forceinline__ int xcc_id() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15;  // hwreg(XCC_ID, 0, 4)
+}
+
+constexpr int kXccProbe = 64;
+__device__ int g_xcc_probe[kXccProbe];
+__global__ __launch_bounds__(64) void xcc_probe_kernel() {
+    if (threadIdx.x == 0) g_xcc_probe[blockIdx.x] = xcc_id();
+}
+
+__global__ __launch_bounds__(256) void clear_slots_kernel() {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kSlots) g_slot_busy[i] = 0;
+}
 
 // Column windows: the descriptor columns are cut into windows of KS = 16.  A window that is
 // zero in every row of a row tile, or in every row of the column tiles it meets, adds an
@@ -327,8 +360,8 @@ __device__ __forceinline__ double scan_step(double v, double take) {
     return fma(__hiloint2double(shi, slo), take, v);
 }
 
-// Output block: 64 x 64 molecule pairs per work-group, accumulated in LDS (each entry
-// owned by one thread: column molecule t % 64, row molecules 16 (t / 64) .. +15).  Per
+// Output block: 64 x 64 molecule pairs per work-group, accumulated in an accumulator slot
+// in device memory (each entry owned by one thread at a time).  Per
 // label, the row sub-tile sr is multiplied with two column sub-tiles at once (the row
 // operand is staged once for both).  Epilogue per tile, in registers: exp of every
 // Gram entry, then a segmented scan along each wave's 16 columns sums the columns of
@@ -384,7 +417,7 @@ struct WinArgs {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
+__global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
     const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
     const int *__restrict__ src1, int ma1, const double *__restrict__ Xp2,
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
@@ -394,7 +427,7 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     // memory once the Gram products are done
     constexpr int OPS = 2 * TS * LDA, EPI = TS * PP;
     __shared__ double Sh[OPS > EPI ? OPS : EPI];
-    __shared__ double accL[MB * AP];
+    __shared__ int s_slot;
     double *const Abuf = Sh, *const P = Sh;
     __shared__ double n2r[TS], n2c[2][TS];
     __shared__ int rowmol[TS], colmol[2][TS], colseg[2][TS], colinfo[2][TS];
@@ -409,6 +442,17 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
     const int gi = blockIdx.x, G = gridDim.x;
     const int2 ab = order[(G % 8 == 0) ? (gi % 8) * (G / 8) + gi / 8 : gi];
     if (ab.x < 0) return;  // padding of a shorter XCD list
+    if (threadIdx.x == 0) {  // claim a free accumulator slot of this XCD
+        const int base = (xcc_id() & (kXcc - 1)) * kSlotsPerXcc;
+        int sl = (int)((blockIdx.x / kXcc) % kSlotsPerXcc);
+        for (int t = 0; t < kSlotsPerXcc; ++t) {
+            if (atomicCAS(&g_slot_busy[base + sl], 0, 1) == 0) break;
+            sl = sl + 1 == kSlotsPerXcc ? 0 : sl + 1;
+        }
+        s_slot = base + sl;
+    }
+    __syncthreads();
+    double *const accL = g_acc_slots + (size_t)s_slot * (MB * AP);
     const int A = ab.x, B = ab.y;
     const int a0 = A * MB, na = min(MB, nm1 - a0);
     const int b0 = B * MB, nb = min(MB, nm2 - b0);
@@ -588,6 +632,51 @@ __global__ __launch_bounds__(NT, 2) void local_scalar_f64_kernel(
             out[ib * nm1 + ia] = val;
         }
     }
+    __syncthreads();  // every read of the slot is done
+    if (threadIdx.x == 0) atomicExch(&g_slot_busy[s_slot], 0);
+}
+
+// Before a launch of the scalar kernel: check, once per device, that the hardware reports
+// XCC ids 0..kXcc-1 (the slot partition relies on them) and that fewer work-groups than
+// slots of one XCD can be resident on an XCD; then clear the slot flags.
+int check_xcc_ids(hipStream_t stream) {
+    static std::mutex mu;
+    static int checked[64] = {};
+    int dev = 0;
+    KF_HIP_TRY(hipGetDevice(&dev));
+    KF_REQUIRE(dev >= 0 && dev < 64, KF_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (checked[dev]) return KF_OK;
+    hipLaunchKernelGGL(xcc_probe_kernel, dim3(kXccProbe), dim3(64), 0, stream);
+    int h[kXccProbe];
+    KF_HIP_TRY(hipMemcpyFromSymbolAsync(h, HIP_SYMBOL(g_xcc_probe), sizeof(h), 0,
+                                        hipMemcpyDeviceToHost, stream));
+    KF_HIP_TRY(hipStreamSynchronize(stream));
+    unsigned seen = 0;
+    for (int i = 0; i < kXccProbe; ++i) {
+        KF_REQUIRE(h[i] >= 0 && h[i] < kXcc, KF_ERR_HIP, "unexpected XCC id %d", h[i]);
+        seen |= 1u << h[i];
+    }
+    KF_REQUIRE(seen == (1u << kXcc) - 1, KF_ERR_HIP,
+               "work-groups did not spread over %d XCC ids (mask 0x%x)", kXcc, seen);
+    checked[dev] = 1;
+    return KF_OK;
+}
+
+template <int MODE>
+int prepare_acc_slots(hipStream_t stream) {
+    int dev = 0, ncu = 0, per_cu = 0, rc = 0;
+    if ((rc = check_xcc_ids(stream))) return rc;
+    KF_HIP_TRY(hipGetDevice(&dev));
+    KF_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    KF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, local_scalar_f64_kernel<MODE>,
+                                                           NT, 0));
+    KF_REQUIRE((long)(ncu + kXcc - 1) / kXcc * per_cu < kSlotsPerXcc, KF_ERR_HIP,
+               "%d CUs x %d resident work-groups exceed the %d accumulator slots per XCD", ncu,
+               per_cu, kSlotsPerXcc);
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(kSlots / 256), dim3(256), 0, stream);
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
 }
 
 // Device copy of a plan plus the packed rows, their window masks and the row tiles'.
@@ -833,6 +922,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     const int2 *order = nullptr;
     unsigned grid = 0;
     if ((rc = upload_block_order(nblk, nblk, true, stream, &order, &grid))) return rc;
+    if ((rc = prepare_acc_slots<MODE>(stream))) return rc;
     ScopedTimer timer(TM_LOCAL_SYMM, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3(grid), dim3(NT), 0, stream,
                        d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
@@ -919,6 +1009,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     const int2 *order = nullptr;
     unsigned grid = 0;
     if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order, &grid))) return rc;
+    if ((rc = prepare_acc_slots<RECT>(stream))) return rc;
     ScopedTimer timer(TM_LOCAL_RECT, stream);
     hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3(grid),
                        dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
