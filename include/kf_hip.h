@@ -52,6 +52,9 @@ extern "C" {
 #define KF_PREC_F64 0    /* FP64 throughout (the reference's precision)                   */
 #define KF_PREC_BF16X3 1 /* BF16x3 split MFMA contraction, FP32 accumulate/exp, FP32 out  */
 #define KF_PREC_F32 2    /* FP32 (factorisations)                                         */
+#define KF_PREC_I8X6 3   /* FP64-accurate int8 split: each descriptor row as 6 int8 slices */
+                         /* of 7 bits (per-row power-of-two scale), exact int32 MFMA slice */
+                         /* products, FP64 norms/exp/sums/out; entries ~1e-11 relative     */
 
 /* Library / device information. */
 int kf_version(void);
@@ -124,18 +127,18 @@ int kf_fchl19v2_f64(const double *coords, const int *Q, const int *N, int nm, in
 
 /* Symmetric kernel written straight into RFP (upper).  Replaces
  * kf::fchl19::kernel_gaussian_symm_rfp (src/local_kernels.cpp:403).
- * precision KF_PREC_F64: arf is double*; KF_PREC_BF16X3: arf is float*. */
+ * precision KF_PREC_F64 or KF_PREC_I8X6: arf is double*; KF_PREC_BF16X3: arf is float*. */
 int kf_local_kernel_symm_rfp(int precision, const double *X, const int *Q, const int *N, int nm,
                              int max_atoms, int rep, double sigma, void *arf, void *stream);
 
 /* Symmetric kernel, dense (nm, nm) row-major, both triangles.  Replaces
  * kf::fchl19::kernel_gaussian_symm (src/local_kernels.cpp:584).  K is double*
- * (KF_PREC_F64) or float* (KF_PREC_BF16X3). */
+ * (KF_PREC_F64, KF_PREC_I8X6) or float* (KF_PREC_BF16X3). */
 int kf_local_kernel_symm(int precision, const double *X, const int *Q, const int *N, int nm,
                          int max_atoms, int rep, double sigma, void *K, void *stream);
 
 /* Rectangular kernel (nm1, nm2) row-major.  Replaces kf::fchl19::kernel_gaussian
- * (src/local_kernels.cpp:155).  K is double* (F64) or float* (BF16X3). */
+ * (src/local_kernels.cpp:155).  K is double* (F64, I8X6) or float* (BF16X3). */
 int kf_local_kernel(int precision, const double *X1, const double *X2, const int *Q1,
                     const int *Q2, const int *N1, const int *N2, int nm1, int nm2,
                     int max_atoms1, int max_atoms2, int rep, double sigma, void *K, void *stream);

@@ -23,7 +23,7 @@ _LIB = None
 KF_OK = 0
 KF_ERR_ARG, KF_ERR_SIGMA, KF_ERR_ELEMENT, KF_ERR_BASIS = -1, -2, -3, -4
 KF_ERR_HIP, KF_ERR_NOMEM, KF_ERR_NOCONV = -5, -6, -7
-KF_PREC_F64, KF_PREC_BF16X3, KF_PREC_F32 = 0, 1, 2
+KF_PREC_F64, KF_PREC_BF16X3, KF_PREC_F32, KF_PREC_I8X6 = 0, 1, 2, 3
 
 _c_int, _c_double, _c_char, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
 

@@ -156,6 +156,82 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
     }
 }
 
+// The int8-split form (precision KF_PREC_I8X6).  A packed row x with mu = 2^e > max|x|
+// (mu = 1 for a zero row) is written as
+//     x = mu * sum_{s < NS} 2^-(6 + 7 s) q_s  +  mu * r,   |r| <= 2^-(7 NS),
+// each integer slice q_s in [-64, 64] the rounded remainder of the ones before it (every
+// step is exact in FP64).  A Gram entry is then the exact integer dot products of slice
+// pairs (s, t) with s + t < NS, summed over each anti-diagonal d = s + t in int32 on the
+// MFMA and combined in FP64 as mu_x mu_y sum_d 2^-(12 + 7 d) acc_d.  Dropping the pairs
+// with s + t >= NS leaves an error near 2^-(7 NS + 5) mu_x mu_y per summed column; with
+// NS = 6 that is kernel entries within ~1e-11 relative on FCHL19 descriptors.
+// Layout: rows of nwin4 windows (16 columns each, nwin4 a multiple of 4, the columns past
+// rep zero), each window its NS slices of 16 bytes (padding windows to 128-byte lines
+// fetched more, not less); one more all-zero row follows the last (row index `rows`,
+// written by the last wave).
+typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr int kI8Slices = 6;
+constexpr int kI8WinBytes = kI8Slices * 16;
+
+template <int NS>
+__global__ __launch_bounds__(256) void pack_rows_i8(const double *__restrict__ X,
+                                                    const int *__restrict__ src,
+                                                    const int *__restrict__ perm, int ma,
+                                                    int rows, int rep, int nwin4,
+                                                    signed char *__restrict__ Sp,
+                                                    double *__restrict__ n2,
+                                                    double *__restrict__ mu,
+                                                    unsigned long long *__restrict__ rowmask) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ncol = nwin4 * 16;
+    if (row == rows) {  // the zero row
+        int *z = reinterpret_cast<int *>(Sp + (size_t)rows * nwin4 * kI8WinBytes);
+        for (int k = lane; k < nwin4 * kI8WinBytes / 4; k += 64) z[k] = 0;
+    }
+    if (row >= rows) return;
+    const int sr = src[row];
+    const size_t xrow = perm ? (size_t)perm[sr / ma] * ma + sr % ma : (size_t)sr;
+    const double *s = X + xrow * rep;
+    double acc = 0.0, mx = 0.0;
+    unsigned long long w0 = 0, w1 = 0;
+    for (int j = 0; j * 64 < ncol; ++j) {
+        const int k = j * 64 + lane;
+        const double v = k < rep ? s[k] : 0.0;
+        acc += v * v;
+        mx = fmax(mx, fabs(v));
+        const unsigned long long w = slice_windows(__ballot(v != 0.0));
+        if (j < 16) w0 |= w << (4 * j);
+        else if (j < kMaxWin / 4) w1 |= w << (4 * j - 64);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc += __shfl_xor(acc, off);
+        mx = fmax(mx, __shfl_xor(mx, off));
+    }
+    int e = 0;
+    frexp(mx, &e);  // mx = f 2^e with f in [1/2, 1), so |x| < 2^e
+    const double inv = mx > 0.0 ? ldexp(1.0, -e) : 1.0;
+    signed char *d = Sp + (size_t)row * nwin4 * kI8WinBytes;
+    for (int j = 0; j * 64 < ncol; ++j) {
+        const int k = j * 64 + lane;
+        double r = (k < rep ? s[k] : 0.0) * inv;  // |r| < 1
+        signed char *p = d + (k >> 4) * kI8WinBytes + (k & 15);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const double sc = (double)(1ll << (6 + 7 * q)), isc = 1.0 / sc;
+            const double t = rint(r * sc);  // in [-64, 64]: |r| <= 2^-(7 q) (q > 0: 1/2 unit)
+            r = fma(-t, isc, r);            // exact: a multiple of ulp(r), below |r|
+            p[q * 16] = (signed char)(int)t;
+        }
+    }
+    if (lane == 0) {
+        n2[row] = acc;
+        mu[row] = mx > 0.0 ? ldexp(1.0, e) : 1.0;
+        rowmask[2 * (size_t)row] = w0;
+        rowmask[2 * (size_t)row + 1] = w1;
+    }
+}
+
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
     unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
     for (int off = 32; off > 0; off >>= 1) {
@@ -349,6 +425,156 @@ __device__ __forceinline__ void gram_dispatch(double *Abuf, const GramArgs &g, i
     }
 }
 
+// Int8-split Gram products of one row sub-tile (64 packed rows) with one column sub-tile,
+// as v_mfma_i32_16x16x64_i8 over chunks of 4 active windows (64 columns).  Lane group
+// k = lane / 16 of both operands carries window 4c + k of chunk c: a permutation of the
+// summation index shared by the two operands, so the integer dot products are exact
+// whatever order the instruction sums its 64 products in.  acc[x][d] (d = s + t) collects
+// the slice pairs of one anti-diagonal: |acc| <= NS * 64^2 * (columns), checked on the
+// host against int32.  Wave w owns columns 16w .. 16w+15; D row 4 (lane / 16) + v, column
+// lane % 16.
+//  * The row chunk (64 rows x 4 windows x NS slices, NS * 4 KB) goes from global memory
+//    straight into one of two LDS buffers (16-byte LDS-destination loads, one barrier per
+//    chunk).  LDS: slice-major, then row, then window slot, the slot of window k in row r
+//    being k XOR bits 2..3 of r, so a fragment's 16-byte reads are conflict-free.  Each
+//    wave loads 16 rows: lane i writes 16 bytes at (its wave's base) + 16 i, i.e. row
+//    16 w + i / 4, slot i % 4, so it fetches window slot ^ ((i / 16) % 4).
+//  * The column operands (NS x 16 bytes per lane and chunk) stream from global memory
+//    into a two-chunk register ring, one chunk ahead.
+//  * A fragment's NS row slices are read from LDS one fragment ahead of its products.
+// On entry the caller's last barrier has freed both buffers; on exit a barrier has
+// retired every read of them.
+struct GramI8Args {
+    const signed char *pa;  // this lane's staging row (clamped into the tile), window 0
+    const signed char *pb;  // this lane's column (clamped into the tile), window 0
+    const signed char *pz;  // an all-zero packed row of the column set
+    int stk, rdo;           // window slot this lane stages; LDS byte offset of its reads
+    int nsub, w0, w1;       // active windows (a multiple of 4) and their list, as GramArgs
+};
+
+// s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8; the others
+// at their maxima): vmcnt(NS) leaves only the NS column loads issued after the row chunk
+// outstanding.
+constexpr int kVmcnt0 = 0x0F70;
+constexpr int kVmcntNs = 0x0F70 | (kI8Slices & 0xF) | ((kI8Slices >> 4) << 14);
+
+// Window of active entry 4c + k (c uniform across the wave, k per lane).
+__device__ __forceinline__ int win_i8(const GramI8Args &g, int c, int k) {
+    const int e = 4 * c + k;
+    return c >= kMaxWin / 4 ? e : __shfl(c < 16 ? g.w0 : g.w1, e & 63);
+}
+
+// Row chunk c into the LDS buffer buf: wave w fills bytes [q 4096 + w 1024, +1024) of
+// every slice q.
+template <int NS>
+__device__ __forceinline__ void stage_rows_i8(signed char *buf, const GramI8Args &g, int c) {
+    const signed char *src = g.pa + win_i8(g, c, g.stk) * kI8WinBytes;
+    signed char *dst = buf + (threadIdx.x >> 6) * 1024;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + 16 * q),
+            (__attribute__((address_space(3))) void *)(dst + q * 4096), 16, 0, 0);
+}
+
+// One chunk: products from buffer cb while the next chunk lands in buffer nb (two distinct
+// LDS arrays, so the compiler sees that the loads into nb do not touch cb).
+template <int NS, int NA, bool HB>
+__device__ __forceinline__ void gram_i8_step(const signed char *__restrict__ cb,
+                                             signed char *__restrict__ nb, const GramI8Args &g,
+                                             int ch, int nch, v4i (&cur)[NS], v4i (&nxt)[NS],
+                                             v4i (&acc)[4][NS]) {
+    const int cn = min(ch + 1, nch - 1);  // the last chunk re-reads itself (never used)
+    v4i a0[NS], a1[NS];
+    __builtin_amdgcn_sched_barrier(0);
+    if (HB) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) a0[q] = *(const v4i *)(cb + q * 4096 + g.rdo);
+    }
+    stage_rows_i8<NS>(nb, g, cn);
+    // (the column loads issue after the row chunk: the wait before the barrier counts on it)
+    __builtin_amdgcn_sched_barrier(0);
+    if (HB) {  // past the last chunk: zeros, so the padding step adds exact zeros
+        const int w = win_i8(g, cn, (threadIdx.x & 63) >> 4);
+        const signed char *pb = ch + 1 < nch ? g.pb + w * kI8WinBytes : g.pz;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) nxt[q] = *(const v4i *)(pb + 16 * q);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (HB) {
+#pragma unroll
+        for (int x = 0; x < NA; ++x) {
+            v4i(&ac)[NS] = (x & 1) ? a1 : a0;
+            v4i(&an)[NS] = (x & 1) ? a0 : a1;
+            if (x + 1 < NA) {
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    an[q] = *(const v4i *)(cb + q * 4096 + (x + 1) * 1024 + g.rdo);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+#pragma unroll
+                for (int p = 0; p < NS - q; ++p)
+                    acc[x][q + p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ac[q], cur[p],
+                                                                          acc[x][q + p], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // the next chunk's rows have landed in LDS (this wave's loads; the barrier covers the
+    // other waves')
+    __builtin_amdgcn_s_waitcnt(HB ? kVmcntNs : kVmcnt0);
+    __syncthreads();
+}
+
+template <int NS, int NA, bool HB>
+__device__ void gram_i8_tile(signed char *__restrict__ buf0, signed char *__restrict__ buf1,
+                             const GramI8Args &g, v4i (&acc)[4][NS]) {
+    const int nch = g.nsub >> 2;
+    if (nch == 0) {  // no window is active: the Gram block is exactly zero
+        __syncthreads();
+        return;
+    }
+    // chunks in pairs, both steps unconditional (an odd count ends with a padding step whose
+    // column operand is zero): a conditional step would let the compiler sink the column
+    // loads of the step before it into the branch, past their prefetch point
+    v4i b0[NS], b1[NS];
+    stage_rows_i8<NS>(buf0, g, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (HB) {
+        const signed char *pb = g.pb + win_i8(g, 0, (threadIdx.x & 63) >> 4) * kI8WinBytes;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) b0[q] = *(const v4i *)(pb + 16 * q);
+    }
+    __builtin_amdgcn_s_waitcnt(HB ? kVmcntNs : kVmcnt0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch += 2) {
+        gram_i8_step<NS, NA, HB>(buf0, buf1, g, ch, nch, b0, b1, acc);
+        gram_i8_step<NS, NA, HB>(buf1, buf0, g, ch + 1, nch, b1, b0, acc);
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void gram_i8_dispatch(signed char *b0, signed char *b1,
+                                                 const GramI8Args &g, int na16, bool hb,
+                                                 v4i (&acc)[4][NS]) {
+#define KF_I8_CASE(n)                                          \
+    case n:                                                    \
+        if (hb) gram_i8_tile<NS, n, true>(b0, b1, g, acc);     \
+        else gram_i8_tile<NS, n, false>(b0, b1, g, acc);       \
+        break;
+    switch (na16) {
+        KF_I8_CASE(1)
+        KF_I8_CASE(2)
+        KF_I8_CASE(3)
+        default:
+            if (hb) gram_i8_tile<NS, 4, true>(b0, b1, g, acc);
+            else gram_i8_tile<NS, 4, false>(b0, b1, g, acc);
+            break;
+    }
+#undef KF_I8_CASE
+}
+
 // v + take * (value of the lane D places lower in the same 16-lane row), take = 0 or 1:
 // one step of a segmented inclusive scan along the 16 columns a row of MFMA output spans
 // (exact: the product is the shifted value or zero).
@@ -387,24 +613,27 @@ struct TileCol {
 // read back, columns past it are the last lanes of a row and never feed a segment of the
 // tile (a scan only takes from lower lanes of its own segment), and lanes that hold no
 // segment tail store to the spare last slot of their row, which no segment uses.  n2r holds
-// cy * |x|^2 of the rows.
-template <bool DS>
+// cy * |x|^2 of the rows.  I8: c is in the int32 MFMA layout (row 4 (lane / 16) + v) and
+// scaled: the Gram entry is c * mur[row] * (this lane's column scale, folded into m2cy).
+template <bool DS, bool I8>
 __device__ __forceinline__ void tile_epilogue(const v4d (&c)[4], int lane, const TileCol &tc,
-                                              const double *n2r, const int *rowmol,
-                                              const double *etab, double m2cy, double *P) {
-    double *const pl = P + (lane >> 4) * PP + tc.slot;
+                                              const double *n2r, const double *mur,
+                                              const int *rowmol, const double *etab, double m2cy,
+                                              double *P) {
+    double *const pl = P + (I8 ? 4 * (lane >> 4) : (lane >> 4)) * PP + tc.slot;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = x * 16 + mfma16_row_f64(lane, v);
-            double e = exp_scaled(fma(c[x][v], m2cy, n2r[row] + tc.n2), etab);
+            const int row = x * 16 + (I8 ? mfma16_row_f32(lane, v) : mfma16_row_f64(lane, v));
+            const double g = I8 ? c[x][v] * mur[row] : c[x][v];
+            double e = exp_scaled(fma(g, m2cy, n2r[row] + tc.n2), etab);
             if (DS && rowmol[row] == tc.mol) e *= 2.0;
             e = scan_step<1>(e, tc.t1);
             e = scan_step<2>(e, tc.t2);
             e = scan_step<4>(e, tc.t4);
             e = scan_step<8>(e, tc.t8);
-            pl[(x * 16 + 4 * v) * PP] = e;  // row x*16 + 4v + lane/16
+            pl[(x * 16 + (I8 ? v : 4 * v)) * PP] = e;  // row x*16 + (the row above)
         }
     }
 }
@@ -413,23 +642,31 @@ struct WinArgs {
     const ulonglong2 *tm1, *tm2;  // tile window masks (null: every window active)
     const int *tb1, *tb2;         // first tile mask of (label, block), l * nblk + block
     const int *perm1, *perm2;     // output index of each molecule (null: identity)
+    const double *mu1, *mu2;      // int8-split form: the rows' scales (null: FP64 form)
+    const signed char *zero2;     // int8-split form: an all-zero packed row of set 2
     int nblk1, nblk2, nwin;       // blocks per set, windows of the padded descriptor
 };
 
-template <int MODE>
-__global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
-    const double *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
-    const int *__restrict__ src1, int ma1, const double *__restrict__ Xp2,
+// NS = 0: FP64 operands (rows of rep_pad doubles), two column tiles per row tile;
+// NS > 0: the int8-split form with NS slices (rows of rep_pad bytes, pack_rows_i8), one
+// column tile per row tile (its accumulators take the registers of the second).
+template <int MODE, int NS>
+__global__ __launch_bounds__(NT, NS ? 2 : 3) void local_scalar_f64_kernel(
+    const void *__restrict__ Xp1, const double *__restrict__ n2a, const int *__restrict__ mr1,
+    const int *__restrict__ src1, int ma1, const void *__restrict__ Xp2,
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
     int ma2, int L, int nm1, int nm2, int rep_pad, double cy, const int2 *__restrict__ order,
     const WinArgs wa, double *__restrict__ out) {
     // two row-operand buffers; the segment partial sums of the epilogue reuse the same
     // memory once the Gram products are done
-    constexpr int OPS = 2 * TS * LDA, EPI = TS * PP;
+    // (int8-split form: the two row buffers are two arrays, the first shared with P)
+    constexpr int OPS = NS ? NS * 4096 / 8 : 2 * TS * LDA, EPI = TS * PP;
+    constexpr int NTM = NS ? 1 : 2;  // column tiles per row tile
     __shared__ double Sh[OPS > EPI ? OPS : EPI];
+    __shared__ double Sh2[NS ? NS * 4096 / 8 : 1];
     __shared__ int s_slot;
     double *const Abuf = Sh, *const P = Sh;
-    __shared__ double n2r[TS], n2c[2][TS];
+    __shared__ double n2r[TS], n2c[2][TS], mur[NS ? TS : 1], muc[NS ? TS : 1];
     __shared__ int rowmol[TS], colmol[2][TS], colseg[2][TS], colinfo[2][TS];
     __shared__ int segf[2][MB], segl[2][MB], rstart[TS + 1], nrr_s;
     __shared__ int mrA[MB + 1], mrB[MB + 1];
@@ -463,6 +700,9 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
     ga.lofs = lrow * LDA + (tid & 3) * 4;
     ga.r16 = lane & 15;
     ga.kq = lane >> 4;
+    GramI8Args g8;
+    g8.stk = (lane & 3) ^ ((lane >> 4) & 3);
+    g8.rdo = (lane & 15) * 64 + 16 * ((lane >> 4) ^ (((lane & 15) >> 2) & 3));
     for (int k = tid; k < MB * AP; k += NT) accL[k] = 0.0;
     if (tid < 64) etab[tid] = kExp2Tab64[tid];  // ordered by the label loop's first barrier
 
@@ -477,9 +717,9 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
         const int tbB = wa.tm1 ? wa.tb2[l * wa.nblk2 + B] : 0;
         const int nsr = (ra1 - ra0 + TS - 1) / TS, nsc = (rb1 - rb0 + TS - 1) / TS;
         for (int sr = 0; sr < nsr; ++sr) {
-            for (int sc0 = diag ? sr : 0; sc0 < nsc; sc0 += 2) {
+            for (int sc0 = diag ? sr : 0; sc0 < nsc; sc0 += NTM) {
                 const int r0 = ra0 + sr * TS, rn = min(TS, ra1 - r0);
-                const int ntile = sc0 + 1 < nsc ? 2 : 1;
+                const int ntile = NTM == 2 && sc0 + 1 < nsc ? 2 : 1;
                 int c0[2], cn[2];
                 c0[0] = rb0 + sc0 * TS;
                 cn[0] = min(TS, rb1 - c0[0]);
@@ -491,6 +731,7 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
                     const int m = v ? src1[r0 + lane] / ma1 - a0 : -1;
                     rowmol[lane] = m;
                     n2r[lane] = v ? cy * n2a[r0 + lane] : 0.0;
+                    if (NS) mur[lane] = v ? wa.mu1[r0 + lane] : 0.0;
                     const int prev = __shfl_up(m, 1, 64);
                     const bool rh = v && (lane == 0 || prev != m);
                     const unsigned long long rheads = __ballot(rh);
@@ -505,6 +746,7 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
                     segf[u][lane] = -1;  // same wave: ordered before the writes below
                     const int m = v ? src2[c0[u] + lane] / ma2 - b0 : -1;
                     n2c[u][lane] = v ? cy * n2b[c0[u] + lane] : 0.0;
+                    if (NS && u == 0) muc[lane] = v ? wa.mu2[c0[0] + lane] : 0.0;
                     const int prev = __shfl_up(m, 1, 64), next = __shfl_down(m, 1, 64);
                     const bool mhead = v && (lane == 0 || prev != m);
                     const bool mtail = v && (lane == 63 || next != m);
@@ -529,11 +771,10 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
                     for (int x = 0; x < 4; ++x) c[u][x] = (v4d){0, 0, 0, 0};
-                ga.pa = Xp1 + (size_t)(r0 + min(lrow, rn - 1)) * rep_pad + (tid & 3) * 4;
-                ga.pb0 = Xp2 + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad + 4 * ga.kq;
-                ga.pb1 = ntile == 2 ? Xp2 + (size_t)(c0[1] + min(bcol, cn[1] - 1)) * rep_pad + 4 * ga.kq
-                                    : ga.pb0;
-                // active windows: non-zero in the row tile and in one of the column tiles
+                // active windows: non-zero in the row tile and in one of the column tiles,
+                // padded with inactive ones (they add exact zeros) to a whole number of
+                // MFMA steps
+                constexpr int WQ = NS ? 4 : 2;
                 unsigned long long m0 = ~0ull, m1 = ~0ull;
                 int nsub = wa.nwin;
                 if (wa.tm1) {
@@ -542,28 +783,61 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
                     m0 = ta.x & (tc0.x | tc1.x);
                     m1 = ta.y & (tc0.y | tc1.y);
                     nsub = __popcll(m0) + __popcll(m1);
-                    if (nsub & 1) {  // even count: add an inactive window (it adds exact zeros)
-                        const int nw = wa.nwin;
-                        const unsigned long long v0 = nw >= 64 ? ~0ull : (1ull << nw) - 1;
-                        const unsigned long long v1 = nw >= 128 ? ~0ull : (nw > 64 ? (1ull << (nw - 64)) - 1 : 0ull);
+                    const int nw = wa.nwin;
+                    const unsigned long long v0 = nw >= 64 ? ~0ull : (1ull << nw) - 1;
+                    const unsigned long long v1 = nw >= 128 ? ~0ull : (nw > 64 ? (1ull << (nw - 64)) - 1 : 0ull);
+                    for (; nsub % WQ; ++nsub) {
                         const unsigned long long z0 = ~m0 & v0, z1 = ~m1 & v1;
                         if (z0) m0 |= z0 & (~z0 + 1);
                         else m1 |= z1 & (~z1 + 1);
-                        ++nsub;
                     }
                 }
+                int w0, w1;
                 {
                     const int n0 = __popcll(m0);
-                    ga.w0 = lane < n0 ? nth_set_bit(m0, lane) : 64 + nth_set_bit(m1, lane - n0);
-                    ga.w1 = 64 + nth_set_bit(m1, 64 + lane - n0);
-                    ga.nsub = nsub;
+                    w0 = lane < n0 ? nth_set_bit(m0, lane) : 64 + nth_set_bit(m1, lane - n0);
+                    w1 = 64 + nth_set_bit(m1, 64 + lane - n0);
                 }
                 const int na16 = (rn + 15) >> 4;
-                const int ntw = (wave * 16 < cn[1]) ? 2 : (wave * 16 < cn[0] ? 1 : 0);
-                if (ntile == 2)
-                    gram_dispatch<2>(Abuf, ga, na16, ntw, c);
-                else
-                    gram_dispatch<1>(Abuf, ga, na16, ntw, c);
+                if constexpr (NS == 0) {
+                    ga.pa = static_cast<const double *>(Xp1) + (size_t)(r0 + min(lrow, rn - 1)) * rep_pad + (tid & 3) * 4;
+                    ga.pb0 = static_cast<const double *>(Xp2) + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad + 4 * ga.kq;
+                    ga.pb1 = ntile == 2 ? static_cast<const double *>(Xp2) + (size_t)(c0[1] + min(bcol, cn[1] - 1)) * rep_pad + 4 * ga.kq
+                                        : ga.pb0;
+                    ga.w0 = w0;
+                    ga.w1 = w1;
+                    ga.nsub = nsub;
+                    const int ntw = (wave * 16 < cn[1]) ? 2 : (wave * 16 < cn[0] ? 1 : 0);
+                    if (ntile == 2)
+                        gram_dispatch<2>(Abuf, ga, na16, ntw, c);
+                    else
+                        gram_dispatch<1>(Abuf, ga, na16, ntw, c);
+                } else {
+                    g8.pa = static_cast<const signed char *>(Xp1) + (size_t)(r0 + min(lrow, rn - 1)) * rep_pad;
+                    g8.pz = wa.zero2;
+                    g8.pb = static_cast<const signed char *>(Xp2) + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad;
+                    g8.w0 = w0;
+                    g8.w1 = w1;
+                    g8.nsub = nsub;
+                    v4i acc[4][NS ? NS : 1];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x)
+#pragma unroll
+                        for (int d = 0; d < NS; ++d) acc[x][d] = (v4i){0, 0, 0, 0};
+                    gram_i8_dispatch<NS>(reinterpret_cast<signed char *>(Sh),
+                                         reinterpret_cast<signed char *>(Sh2), g8, na16,
+                                         wave * 16 < cn[0], acc);
+                    // sum_d 2^-7d acc_d in FP64 (the common 2^-12 and the scales in the epilogue)
+#pragma unroll
+                    for (int x = 0; x < 4; ++x)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            double t = (double)acc[x][NS - 1][v];
+#pragma unroll
+                            for (int d = NS - 2; d >= 0; --d) t = fma(t, 0x1p-7, (double)acc[x][d][v]);
+                            c[0][x][v] = t;
+                        }
+                }
 
                 // ---- per tile: exp, segment sums in registers, row molecules ----
 #pragma unroll
@@ -579,10 +853,12 @@ __global__ __launch_bounds__(NT, 3) void local_scalar_f64_kernel(
                                          dist >= 1 ? 1.0 : 0.0, dist >= 2 ? 1.0 : 0.0,
                                          dist >= 4 ? 1.0 : 0.0, dist >= 8 ? 1.0 : 0.0,
                                          colmol[u][col], tail ? colseg[u][col] : PP - 1};
+                        // int8-split form: the column's scale and the common 2^-12 here
+                        const double m2 = NS ? -2.0 * cy * 0x1p-12 * muc[col] : -2.0 * cy;
                         if (dbl_self)
-                            tile_epilogue<true>(c[u], lane, tc, n2r, rowmol, etab, -2.0 * cy, P);
+                            tile_epilogue<true, NS != 0>(c[u], lane, tc, n2r, mur, rowmol, etab, m2, P);
                         else
-                            tile_epilogue<false>(c[u], lane, tc, n2r, rowmol, etab, -2.0 * cy, P);
+                            tile_epilogue<false, NS != 0>(c[u], lane, tc, n2r, mur, rowmol, etab, m2, P);
                     }
                     __syncthreads();
                     // thread (wave w, lane b) adds, for column molecule b, every 4th row
@@ -663,14 +939,14 @@ int check_xcc_ids(hipStream_t stream) {
     return KF_OK;
 }
 
-template <int MODE>
+template <int MODE, int NS>
 int prepare_acc_slots(hipStream_t stream) {
     int dev = 0, ncu = 0, per_cu = 0, rc = 0;
     if ((rc = check_xcc_ids(stream))) return rc;
     KF_HIP_TRY(hipGetDevice(&dev));
     KF_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    KF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, local_scalar_f64_kernel<MODE>,
-                                                           NT, 0));
+    KF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, local_scalar_f64_kernel<MODE, NS>, NT, 0));
     KF_REQUIRE((long)(ncu + kXcc - 1) / kXcc * per_cu < kSlotsPerXcc, KF_ERR_HIP,
                "%d CUs x %d resident work-groups exceed the %d accumulator slots per XCD", ncu,
                per_cu, kSlotsPerXcc);
@@ -680,16 +956,24 @@ int prepare_acc_slots(hipStream_t stream) {
 }
 
 // Device copy of a plan plus the packed rows, their window masks and the row tiles'.
+// FP64 form (ns = 0): Xp rows of rep_pad doubles; int8-split form (ns slices): Xp rows of
+// rep_pad bytes (nwin windows of kI8WinBytes, pack_rows_i8) plus a zero row, and mu the
+// rows' scales.
 struct DevicePacked {
-    double *Xp = nullptr, *n2 = nullptr;
+    void *Xp = nullptr;
+    double *n2 = nullptr, *mu = nullptr;
     int *mol_row = nullptr, *src = nullptr, *perm = nullptr, *tbase = nullptr;
     unsigned long long *rowmask = nullptr;
     ulonglong2 *tmask = nullptr;  // null: windows not masked
-    int rows = 0, rep_pad = 0, nblk = 0;
+    int rows = 0, rep_pad = 0, nblk = 0, nwin = 0;
 };
 
 inline int padded_rep(int rep) { return (rep + KA - 1) / KA * KA; }
-inline bool windows_masked(int rep) { return padded_rep(rep) / KS <= kMaxWin; }
+// windows of the int8-split rows: a multiple of 4 (one MFMA step)
+inline int i8_windows(int rep) { return (rep + 63) / 64 * 4; }
+inline bool windows_masked(int rep, int ns) {
+    return (ns ? i8_windows(rep) : padded_rep(rep) / KS) <= kMaxWin;
+}
 
 // Window masks of every molecule of a set (device X) -> host (two words per molecule).
 int molecule_windows(const double *X, const int *N, int nm, int ma, int rep, int ws,
@@ -767,11 +1051,12 @@ void permute_metadata(const std::vector<int> &Q, const std::vector<int> &N, int 
 // Upload the plan (plus the molecule order, if any), pack the rows and, when the windows
 // are masked, build the masks of the kernel's row tiles.
 int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const double *X, int rep,
-                    int ws_pack, int ws_plan, hipStream_t stream, DevicePacked &d) {
+                    int ns, int ws_pack, int ws_plan, hipStream_t stream, DevicePacked &d) {
     d.rows = p.rows;
-    d.rep_pad = padded_rep(rep);
+    d.nwin = ns ? i8_windows(rep) : padded_rep(rep) / KS;
+    d.rep_pad = ns ? d.nwin * kI8WinBytes : padded_rep(rep);
     d.nblk = (p.nm + MB - 1) / MB;
-    const bool masked = windows_masked(rep);
+    const bool masked = windows_masked(rep, ns);
     const size_t mr_n = p.mol_row.size(), src_n = p.src.size(), perm_n = perm.size();
     const size_t nla = (size_t)p.nlabels * d.nblk;
     // tile-mask bases from the host plan: tiles of (label, block) in the kernel's tiling
@@ -785,7 +1070,10 @@ int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const doub
     const size_t ntile_masks = masked ? (size_t)tbase[nla] : 0;
     const size_t plan_n = mr_n + src_n + perm_n + nla + 1;
     char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * plan_n, stream));
-    const size_t xp_bytes = sizeof(double) * ((size_t)p.rows * d.rep_pad + p.rows + 64);
+    // operand rows, then n2 (and mu) of the rows
+    const size_t op_bytes = ((ns ? (size_t)(p.rows + 1) * d.rep_pad
+                                 : sizeof(double) * (size_t)p.rows * d.rep_pad) + 255) / 256 * 256;
+    const size_t xp_bytes = op_bytes + sizeof(double) * (2 * (size_t)p.rows + 64);
     const size_t rm_off = (xp_bytes + 255) / 256 * 256;
     const size_t tm_off = rm_off + (2 * sizeof(unsigned long long) * (size_t)p.rows + 255) / 256 * 256;
     const size_t pack_bytes = tm_off + sizeof(ulonglong2) * (ntile_masks + 1);
@@ -803,13 +1091,20 @@ int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const doub
     std::copy(perm.begin(), perm.end(), stage + mr_n + src_n);
     std::copy(tbase.begin(), tbase.end(), stage + mr_n + src_n + perm_n);
     KF_HIP_TRY(hipMemcpyAsync(plan, stage, sizeof(int) * plan_n, hipMemcpyHostToDevice, stream));
-    d.Xp = reinterpret_cast<double *>(pack);
-    d.n2 = d.Xp + (size_t)p.rows * d.rep_pad;
+    d.Xp = pack;
+    d.n2 = reinterpret_cast<double *>(pack + op_bytes);
+    d.mu = ns ? d.n2 + p.rows : nullptr;
     d.rowmask = reinterpret_cast<unsigned long long *>(pack + rm_off);
     d.tmask = masked ? reinterpret_cast<ulonglong2 *>(pack + tm_off) : nullptr;
     if (p.rows > 0) {
-        hipLaunchKernelGGL(pack_rows_f64, dim3((p.rows + 3) / 4), dim3(256), 0, stream, X, d.src,
-                           d.perm, p.max_atoms, p.rows, rep, d.rep_pad, d.Xp, d.n2, d.rowmask);
+        if (ns == kI8Slices)  // (one more wave writes the zero row)
+            hipLaunchKernelGGL(pack_rows_i8<kI8Slices>, dim3((p.rows + 4) / 4), dim3(256), 0,
+                               stream, X, d.src, d.perm, p.max_atoms, p.rows, rep, d.nwin,
+                               static_cast<signed char *>(d.Xp), d.n2, d.mu, d.rowmask);
+        else
+            hipLaunchKernelGGL(pack_rows_f64, dim3((p.rows + 3) / 4), dim3(256), 0, stream, X,
+                               d.src, d.perm, p.max_atoms, p.rows, rep, d.rep_pad,
+                               static_cast<double *>(d.Xp), d.n2, d.rowmask);
         KF_HIP_TRY(hipGetLastError());
         if (masked && nla > 0) {
             hipLaunchKernelGGL(tile_mask_kernel, dim3((unsigned)nla), dim3(256), 0, stream,
@@ -828,9 +1123,13 @@ WinArgs win_args(const DevicePacked &d1, const DevicePacked &d2) {
     w.tb2 = d2.tbase;
     w.perm1 = d1.perm;
     w.perm2 = d2.perm;
+    w.mu1 = d1.mu;
+    w.mu2 = d2.mu;
+    w.zero2 = d2.mu ? static_cast<const signed char *>(d2.Xp) + (size_t)d2.rows * d2.rep_pad
+                    : nullptr;
     w.nblk1 = d1.nblk;
     w.nblk2 = d2.nblk;
-    w.nwin = d1.rep_pad / KS;
+    w.nwin = d1.nwin;
     return w;
 }
 
@@ -875,6 +1174,34 @@ int upload_block_order(int nb1, int nb2, bool symm, hipStream_t stream, const in
     return KF_OK;
 }
 
+// Launch of the scalar kernel in its FP64 form (ns = 0) or int8-split form (ns slices).
+template <int MODE>
+int launch_scalar(int ns, const DevicePacked &d1, int ma1, const DevicePacked &d2, int ma2, int L,
+                  int nm1, int nm2, double cy, const int2 *order, unsigned grid, double *out,
+                  int timer, hipStream_t stream) {
+    int rc = 0;
+    if (ns == kI8Slices) {
+        // int32 anti-diagonal sums: ns slice pairs of |q| <= 64 over the padded columns
+        KF_REQUIRE((long long)ns * d1.nwin * 16 * 64 * 64 < (1ll << 31), KF_ERR_ARG,
+                   "descriptor too wide (%d windows) for the int8-split form", d1.nwin);
+        if ((rc = prepare_acc_slots<MODE, kI8Slices>(stream))) return rc;
+        ScopedTimer t(timer, stream);
+        hipLaunchKernelGGL((local_scalar_f64_kernel<MODE, kI8Slices>), dim3(grid), dim3(NT), 0,
+                           stream, d1.Xp, d1.n2, d1.mol_row, d1.src, ma1, d2.Xp, d2.n2,
+                           d2.mol_row, d2.src, ma2, L, nm1, nm2, d1.rep_pad, cy, order,
+                           win_args(d1, d2), out);
+        KF_HIP_TRY(hipGetLastError());
+        return KF_OK;
+    }
+    if ((rc = prepare_acc_slots<MODE, 0>(stream))) return rc;
+    ScopedTimer t(timer, stream);
+    hipLaunchKernelGGL((local_scalar_f64_kernel<MODE, 0>), dim3(grid), dim3(NT), 0, stream, d1.Xp,
+                       d1.n2, d1.mol_row, d1.src, ma1, d2.Xp, d2.n2, d2.mol_row, d2.src, ma2, L,
+                       nm1, nm2, d1.rep_pad, cy, order, win_args(d1, d2), out);
+    KF_HIP_TRY(hipGetLastError());
+    return KF_OK;
+}
+
 int check_common(const double *X, const int *Q, const int *N, int nm, int ma, int rep,
                  double sigma) {
     KF_REQUIRE(nm >= 0 && ma >= 0 && rep >= 1, KF_ERR_ARG, "bad dimensions");
@@ -888,8 +1215,10 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
               double sigma, void *out, hipStream_t stream) {
     int rc = check_common(X, Q, N, nm, ma, rep, sigma);
     if (rc) return rc;
-    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3, KF_ERR_ARG,
-               "precision %d not supported here", precision);
+    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3 ||
+                   precision == KF_PREC_I8X6,
+               KF_ERR_ARG, "precision %d not supported here", precision);
+    const int ns = precision == KF_PREC_I8X6 ? kI8Slices : 0;
     if (nm == 0) return KF_OK;
     KF_REQUIRE(out, KF_ERR_ARG, "null output");
     std::vector<int> hQ, hN;
@@ -901,7 +1230,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
         return local_mixed_symm(MODE == SYMM_DENSE, X, nm, ma, rep, sigma, p, out, stream);
     }
     std::vector<int> perm, Qp, Np;
-    if (windows_masked(rep)) {
+    if (windows_masked(rep, ns)) {
         std::vector<unsigned long long> mm;
         if ((rc = molecule_windows(X, N, nm, ma, rep, WS_AUX2, stream, mm))) return rc;
         perm = order_molecules(mm, rank_windows({&mm}, padded_rep(rep) / KS));
@@ -911,7 +1240,7 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
         build_plan(hQ.data(), hN.data(), nm, ma, labels, p);
     }
     DevicePacked d;
-    if ((rc = upload_and_pack(p, perm, X, rep, WS_PACK, WS_PLAN, stream, d))) return rc;
+    if ((rc = upload_and_pack(p, perm, X, rep, ns, WS_PACK, WS_PLAN, stream, d))) return rc;
     const int nblk = (nm + MB - 1) / MB;
     const double cy = -kExpScale / (2.0 * sigma * sigma);  // squared distance -> exp_scaled's y
     if (p.nlabels == 0) {  // no atoms at all: zero output
@@ -922,14 +1251,8 @@ int symm_impl(int precision, const double *X, const int *Q, const int *N, int nm
     const int2 *order = nullptr;
     unsigned grid = 0;
     if ((rc = upload_block_order(nblk, nblk, true, stream, &order, &grid))) return rc;
-    if ((rc = prepare_acc_slots<MODE>(stream))) return rc;
-    ScopedTimer timer(TM_LOCAL_SYMM, stream);
-    hipLaunchKernelGGL(local_scalar_f64_kernel<MODE>, dim3(grid), dim3(NT), 0, stream,
-                       d.Xp, d.n2, d.mol_row, d.src, ma, d.Xp, d.n2, d.mol_row, d.src, ma,
-                       p.nlabels, nm, nm, d.rep_pad, cy, order, win_args(d, d),
-                       static_cast<double *>(out));
-    KF_HIP_TRY(hipGetLastError());
-    return KF_OK;
+    return launch_scalar<MODE>(ns, d, ma, d, ma, p.nlabels, nm, nm, cy, order, grid,
+                               static_cast<double *>(out), TM_LOCAL_SYMM, stream);
 }
 
 }  // namespace
@@ -961,8 +1284,10 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     int rc = check_common(X1, Q1, N1, nm1, max_atoms1, rep, sigma);
     if (rc) return rc;
     if ((rc = check_common(X2, Q2, N2, nm2, max_atoms2, rep, sigma))) return rc;
-    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3, KF_ERR_ARG,
-               "precision %d not supported here", precision);
+    KF_REQUIRE(precision == KF_PREC_F64 || precision == KF_PREC_BF16X3 ||
+                   precision == KF_PREC_I8X6,
+               KF_ERR_ARG, "precision %d not supported here", precision);
+    const int ns = precision == KF_PREC_I8X6 ? kI8Slices : 0;
     if (nm1 == 0 || nm2 == 0) return KF_OK;
     KF_REQUIRE(K, KF_ERR_ARG, "null output");
     std::vector<int> q1, n1, q2, n2;
@@ -972,7 +1297,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
     std::vector<int> l2 = distinct_labels(q2.data(), n2.data(), nm2, max_atoms2);
     std::vector<int> labels;
     std::set_intersection(l1.begin(), l1.end(), l2.begin(), l2.end(), std::back_inserter(labels));
-    const size_t esz = precision == KF_PREC_F64 ? sizeof(double) : sizeof(float);
+    const size_t esz = precision == KF_PREC_BF16X3 ? sizeof(float) : sizeof(double);
     if (labels.empty()) {
         KF_HIP_TRY(hipMemsetAsync(K, 0, esz * (size_t)nm1 * nm2, stream));
         return KF_OK;
@@ -985,7 +1310,7 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
                                 static_cast<float *>(K), stream);
     }
     std::vector<int> perm1, perm2;
-    if (windows_masked(rep)) {
+    if (windows_masked(rep, ns)) {
         std::vector<unsigned long long> m1, m2;
         if ((rc = molecule_windows(X1, N1, nm1, max_atoms1, rep, WS_AUX2, stream, m1))) return rc;
         if ((rc = molecule_windows(X2, N2, nm2, max_atoms2, rep, WS_AUX3, stream, m2))) return rc;
@@ -1002,19 +1327,13 @@ extern "C" int kf_local_kernel(int precision, const double *X1, const double *X2
         build_plan(q2.data(), n2.data(), nm2, max_atoms2, labels, p2);
     }
     DevicePacked d1, d2;
-    if ((rc = upload_and_pack(p1, perm1, X1, rep, WS_PACK, WS_PLAN, stream, d1))) return rc;
-    if ((rc = upload_and_pack(p2, perm2, X2, rep, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
+    if ((rc = upload_and_pack(p1, perm1, X1, rep, ns, WS_PACK, WS_PLAN, stream, d1))) return rc;
+    if ((rc = upload_and_pack(p2, perm2, X2, rep, ns, WS_PACK2, WS_PLAN2, stream, d2))) return rc;
     const int nblk1 = (nm1 + MB - 1) / MB, nblk2 = (nm2 + MB - 1) / MB;
     const double cy = -kExpScale / (2.0 * sigma * sigma);  // squared distance -> exp_scaled's y
     const int2 *order = nullptr;
     unsigned grid = 0;
     if ((rc = upload_block_order(nblk1, nblk2, false, stream, &order, &grid))) return rc;
-    if ((rc = prepare_acc_slots<RECT>(stream))) return rc;
-    ScopedTimer timer(TM_LOCAL_RECT, stream);
-    hipLaunchKernelGGL(local_scalar_f64_kernel<RECT>, dim3(grid),
-                       dim3(NT), 0, stream, d1.Xp, d1.n2, d1.mol_row, d1.src, max_atoms1, d2.Xp,
-                       d2.n2, d2.mol_row, d2.src, max_atoms2, (int)labels.size(), nm1, nm2,
-                       d1.rep_pad, cy, order, win_args(d1, d2), static_cast<double *>(K));
-    KF_HIP_TRY(hipGetLastError());
-    return KF_OK;
+    return launch_scalar<RECT>(ns, d1, max_atoms1, d2, max_atoms2, (int)labels.size(), nm1, nm2,
+                               cy, order, grid, static_cast<double *>(K), TM_LOCAL_RECT, stream);
 }

@@ -20,12 +20,13 @@ struct Set {
 int parse_precision(const std::string &p) {
     if (p == "fp64" || p == "float64") return KF_PREC_F64;
     if (p == "bf16x3") return KF_PREC_BF16X3;
-    throw py::value_error("precision must be 'fp64' or 'bf16x3'");
+    if (p == "i8x6") return KF_PREC_I8X6;
+    throw py::value_error("precision must be 'fp64', 'i8x6' or 'bf16x3'");
 }
 
 // Download a float32 or float64 device result into a NumPy array of that dtype.
 py::array download_result(void *dev, size_t n, int prec, std::vector<py::ssize_t> shape) {
-    if (prec == KF_PREC_F64) {
+    if (prec != KF_PREC_BF16X3) {
         py::array_t<double> out(shape);
         hip_check(hipMemcpy(out.mutable_data(), dev, n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
         return out;
@@ -322,7 +323,9 @@ PYBIND11_MODULE(local_kernels, m) {
           py::arg("Q2"), py::arg("N1"), py::arg("N2"), py::arg("sigma"), py::kw_only(),
           py::arg("precision") = "fp64",
           "K[a,b] = sum over label-matched atom pairs of exp(-|x1-x2|^2/2sigma^2), (nm1, nm2).\n"
-          "precision='bf16x3' (keyword only, an extension): BF16x3 MFMA contraction, float32 out.");
+          "precision (keyword only, an extension): 'i8x6' FP64-accurate int8-split contraction\n"
+          "(six 7-bit slices per row, exact int32 MFMA products; entries ~1e-11 relative, float64\n"
+          "out) or 'bf16x3' BF16x3 MFMA contraction (float32 out).");
     m.def("kernel_gaussian_symm", &kernel_gaussian_symm, py::arg("X"), py::arg("Q"), py::arg("N"),
           py::arg("sigma"), py::kw_only(), py::arg("precision") = "fp64",
           "Symmetric local kernel, dense (nm, nm).");
