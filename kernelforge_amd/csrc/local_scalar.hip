@@ -167,8 +167,7 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
 // NS = 6 that is kernel entries within ~1e-11 relative on FCHL19 descriptors.
 // Layout: rows of nwin4 windows (16 columns each, nwin4 a multiple of 4, the columns past
 // rep zero), each window its NS slices of 16 bytes (padding windows to 128-byte lines
-// fetched more, not less); one more all-zero row follows the last (row index `rows`,
-// written by the last wave).
+// fetched more, not less).
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int kI8Slices = 6;
 constexpr int kI8WinBytes = kI8Slices * 16;
@@ -185,10 +184,6 @@ __global__ __launch_bounds__(256) void pack_rows_i8(const double *__restrict__ X
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int ncol = nwin4 * 16;
-    if (row == rows) {  // the zero row
-        int *z = reinterpret_cast<int *>(Sp + (size_t)rows * nwin4 * kI8WinBytes);
-        for (int k = lane; k < nwin4 * kI8WinBytes / 4; k += 64) z[k] = 0;
-    }
     if (row >= rows) return;
     const int sr = src[row];
     const size_t xrow = perm ? (size_t)perm[sr / ma] * ma + sr % ma : (size_t)sr;
@@ -433,30 +428,36 @@ __device__ __forceinline__ void gram_dispatch(double *Abuf, const GramArgs &g, i
 // the slice pairs of one anti-diagonal: |acc| <= NS * 64^2 * (columns), checked on the
 // host against int32.  Wave w owns columns 16w .. 16w+15; D row 4 (lane / 16) + v, column
 // lane % 16.
-//  * The row chunk (64 rows x 4 windows x NS slices, NS * 4 KB) goes from global memory
-//    straight into one of two LDS buffers (16-byte LDS-destination loads, one barrier per
-//    chunk).  LDS: slice-major, then row, then window slot, the slot of window k in row r
-//    being k XOR bits 2..3 of r, so a fragment's 16-byte reads are conflict-free.  Each
-//    wave loads 16 rows: lane i writes 16 bytes at (its wave's base) + 16 i, i.e. row
-//    16 w + i / 4, slot i % 4, so it fetches window slot ^ ((i / 16) % 4).
-//  * The column operands (NS x 16 bytes per lane and chunk) stream from global memory
-//    into a two-chunk register ring, one chunk ahead.
+//  * Both operands go from global memory straight into LDS (16-byte LDS-destination
+//    loads), one chunk ahead: the row chunk (64 rows x 4 windows x NS slices) into one of
+//    two row buffers shared by the four waves, each wave's column chunk (its 16 columns)
+//    into its own region, read back into registers at the start of the chunk's step.  No
+//    other global loads are in flight in the loop (plain loads share the counter with
+//    the LDS-destination ones and would make the compiler wait for all of them).
+//  * Row buffers: slice-major, then row, then window slot, the slot of window k in row r
+//    being k XOR bits 1..2 of r, so the eight lanes of a fragment read served together
+//    (rows 8j .. 8j+7, 128 bytes) hit eight different 16-byte bank groups.  Each wave
+//    loads 16 rows: lane i writes 16 bytes at (its wave's base) + 16 i, i.e. row
+//    16 w + i / 4, slot i % 4, so it fetches window slot ^ ((i / 8) % 4).
+//  * Column region of a wave: slice q of lane i at 1024 q + 16 i.
 //  * A fragment's NS row slices are read from LDS one fragment ahead of its products.
-// On entry the caller's last barrier has freed both buffers; on exit a barrier has
-// retired every read of them.
+// On entry the caller's last barrier has freed the LDS; on exit a barrier has retired
+// every read of it.
+constexpr int kI8RowBuf = kI8Slices * 4096;  // bytes of one row-chunk buffer
+constexpr int kI8ColReg = kI8Slices * 1024;  // bytes of one wave's column region
+constexpr int kI8Lds = 2 * kI8RowBuf + 4 * kI8ColReg;
+
 struct GramI8Args {
     const signed char *pa;  // this lane's staging row (clamped into the tile), window 0
     const signed char *pb;  // this lane's column (clamped into the tile), window 0
-    const signed char *pz;  // an all-zero packed row of the column set
     int stk, rdo;           // window slot this lane stages; LDS byte offset of its reads
     int nsub, w0, w1;       // active windows (a multiple of 4) and their list, as GramArgs
 };
 
 // s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8; the others
-// at their maxima): vmcnt(NS) leaves only the NS column loads issued after the row chunk
-// outstanding.
+// at their maxima): every global load done; every LDS access done.
 constexpr int kVmcnt0 = 0x0F70;
-constexpr int kVmcntNs = 0x0F70 | (kI8Slices & 0xF) | ((kI8Slices >> 4) << 14);
+constexpr int kLgkmcnt0 = 0xC07F;
 
 // Window of active entry 4c + k (c uniform across the wave, k per lane).
 __device__ __forceinline__ int win_i8(const GramI8Args &g, int c, int k) {
@@ -464,7 +465,7 @@ __device__ __forceinline__ int win_i8(const GramI8Args &g, int c, int k) {
     return c >= kMaxWin / 4 ? e : __shfl(c < 16 ? g.w0 : g.w1, e & 63);
 }
 
-// Row chunk c into the LDS buffer buf: wave w fills bytes [q 4096 + w 1024, +1024) of
+// Row chunk c into the row buffer buf: wave w fills bytes [q 4096 + w 1024, +1024) of
 // every slice q.
 template <int NS>
 __device__ __forceinline__ void stage_rows_i8(signed char *buf, const GramI8Args &g, int c) {
@@ -477,28 +478,37 @@ __device__ __forceinline__ void stage_rows_i8(signed char *buf, const GramI8Args
             (__attribute__((address_space(3))) void *)(dst + q * 4096), 16, 0, 0);
 }
 
-// One chunk: products from buffer cb while the next chunk lands in buffer nb (two distinct
-// LDS arrays, so the compiler sees that the loads into nb do not touch cb).
+// This wave's column chunk c into its region reg.
+template <int NS>
+__device__ __forceinline__ void stage_cols_i8(signed char *reg, const GramI8Args &g, int c) {
+    const signed char *src = g.pb + win_i8(g, c, (threadIdx.x & 63) >> 4) * kI8WinBytes;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + 16 * q),
+            (__attribute__((address_space(3))) void *)(reg + q * 1024), 16, 0, 0);
+}
+
+// One chunk: products from row buffer cb and the column region while the next chunk
+// lands in row buffer nb and (after its reads) the column region.
 template <int NS, int NA, bool HB>
-__device__ __forceinline__ void gram_i8_step(const signed char *__restrict__ cb,
-                                             signed char *__restrict__ nb, const GramI8Args &g,
-                                             int ch, int nch, v4i (&cur)[NS], v4i (&nxt)[NS],
-                                             v4i (&acc)[4][NS]) {
-    const int cn = min(ch + 1, nch - 1);  // the last chunk re-reads itself (never used)
-    v4i a0[NS], a1[NS];
+__device__ __forceinline__ void gram_i8_step(const signed char *cb, signed char *nb,
+                                             signed char *reg, const GramI8Args &g, int ch,
+                                             int nch, v4i (&acc)[4][NS]) {
+    const int lane = threadIdx.x & 63;
+    const bool more = ch + 1 < nch;
+    v4i b[NS], a0[NS], a1[NS];
     __builtin_amdgcn_sched_barrier(0);
     if (HB) {
 #pragma unroll
-        for (int q = 0; q < NS; ++q) a0[q] = *(const v4i *)(cb + q * 4096 + g.rdo);
-    }
-    stage_rows_i8<NS>(nb, g, cn);
-    // (the column loads issue after the row chunk: the wait before the barrier counts on it)
-    __builtin_amdgcn_sched_barrier(0);
-    if (HB) {  // past the last chunk: zeros, so the padding step adds exact zeros
-        const int w = win_i8(g, cn, (threadIdx.x & 63) >> 4);
-        const signed char *pb = ch + 1 < nch ? g.pb + w * kI8WinBytes : g.pz;
+        for (int q = 0; q < NS; ++q) b[q] = *(const v4i *)(reg + q * 1024 + 16 * lane);
 #pragma unroll
-        for (int q = 0; q < NS; ++q) nxt[q] = *(const v4i *)(pb + 16 * q);
+        for (int q = 0; q < NS; ++q) a0[q] = *(const v4i *)(cb + q * 4096 + g.rdo);
+        if (more) __builtin_amdgcn_s_waitcnt(kLgkmcnt0);  // the region is rewritten next
+    }
+    if (more) {
+        stage_rows_i8<NS>(nb, g, ch + 1);
+        if (HB) stage_cols_i8<NS>(reg, g, ch + 1);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (HB) {
@@ -516,60 +526,54 @@ __device__ __forceinline__ void gram_i8_step(const signed char *__restrict__ cb,
             for (int q = 0; q < NS; ++q)
 #pragma unroll
                 for (int p = 0; p < NS - q; ++p)
-                    acc[x][q + p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ac[q], cur[p],
+                    acc[x][q + p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ac[q], b[p],
                                                                           acc[x][q + p], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // the next chunk's rows have landed in LDS (this wave's loads; the barrier covers the
-    // other waves')
-    __builtin_amdgcn_s_waitcnt(HB ? kVmcntNs : kVmcnt0);
+    // the next chunk has landed (this wave's loads; the barrier covers the other waves')
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
     __syncthreads();
 }
 
+// lds: the kernel's shared array (kI8Lds bytes): row buffers 0 and 1, then the column
+// regions of waves 0..3 (one array, constant offsets: the compiler sees that loads into
+// one part do not touch another).
 template <int NS, int NA, bool HB>
-__device__ void gram_i8_tile(signed char *__restrict__ buf0, signed char *__restrict__ buf1,
-                             const GramI8Args &g, v4i (&acc)[4][NS]) {
+__device__ void gram_i8_tile(signed char *__restrict__ lds, const GramI8Args &g,
+                             v4i (&acc)[4][NS]) {
     const int nch = g.nsub >> 2;
     if (nch == 0) {  // no window is active: the Gram block is exactly zero
         __syncthreads();
         return;
     }
-    // chunks in pairs, both steps unconditional (an odd count ends with a padding step whose
-    // column operand is zero): a conditional step would let the compiler sink the column
-    // loads of the step before it into the branch, past their prefetch point
-    v4i b0[NS], b1[NS];
+    signed char *const buf0 = lds, *const buf1 = lds + kI8RowBuf;
+    signed char *const reg = lds + 2 * kI8RowBuf + (threadIdx.x >> 6) * kI8ColReg;
     stage_rows_i8<NS>(buf0, g, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    if (HB) {
-        const signed char *pb = g.pb + win_i8(g, 0, (threadIdx.x & 63) >> 4) * kI8WinBytes;
-#pragma unroll
-        for (int q = 0; q < NS; ++q) b0[q] = *(const v4i *)(pb + 16 * q);
-    }
-    __builtin_amdgcn_s_waitcnt(HB ? kVmcntNs : kVmcnt0);
+    if (HB) stage_cols_i8<NS>(reg, g, 0);
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
     __syncthreads();
     for (int ch = 0; ch < nch; ch += 2) {
-        gram_i8_step<NS, NA, HB>(buf0, buf1, g, ch, nch, b0, b1, acc);
-        gram_i8_step<NS, NA, HB>(buf1, buf0, g, ch + 1, nch, b1, b0, acc);
+        gram_i8_step<NS, NA, HB>(buf0, buf1, reg, g, ch, nch, acc);
+        if (ch + 1 < nch) gram_i8_step<NS, NA, HB>(buf1, buf0, reg, g, ch + 1, nch, acc);
     }
 }
 
 template <int NS>
-__device__ __forceinline__ void gram_i8_dispatch(signed char *b0, signed char *b1,
-                                                 const GramI8Args &g, int na16, bool hb,
-                                                 v4i (&acc)[4][NS]) {
-#define KF_I8_CASE(n)                                          \
-    case n:                                                    \
-        if (hb) gram_i8_tile<NS, n, true>(b0, b1, g, acc);     \
-        else gram_i8_tile<NS, n, false>(b0, b1, g, acc);       \
+__device__ __forceinline__ void gram_i8_dispatch(signed char *lds, const GramI8Args &g, int na16,
+                                                 bool hb, v4i (&acc)[4][NS]) {
+#define KF_I8_CASE(n)                                     \
+    case n:                                               \
+        if (hb) gram_i8_tile<NS, n, true>(lds, g, acc);   \
+        else gram_i8_tile<NS, n, false>(lds, g, acc);     \
         break;
     switch (na16) {
         KF_I8_CASE(1)
         KF_I8_CASE(2)
         KF_I8_CASE(3)
         default:
-            if (hb) gram_i8_tile<NS, 4, true>(b0, b1, g, acc);
-            else gram_i8_tile<NS, 4, false>(b0, b1, g, acc);
+            if (hb) gram_i8_tile<NS, 4, true>(lds, g, acc);
+            else gram_i8_tile<NS, 4, false>(lds, g, acc);
             break;
     }
 #undef KF_I8_CASE
@@ -643,7 +647,6 @@ struct WinArgs {
     const int *tb1, *tb2;         // first tile mask of (label, block), l * nblk + block
     const int *perm1, *perm2;     // output index of each molecule (null: identity)
     const double *mu1, *mu2;      // int8-split form: the rows' scales (null: FP64 form)
-    const signed char *zero2;     // int8-split form: an all-zero packed row of set 2
     int nblk1, nblk2, nwin;       // blocks per set, windows of the padded descriptor
 };
 
@@ -657,13 +660,12 @@ __global__ __launch_bounds__(NT, NS ? 2 : 3) void local_scalar_f64_kernel(
     const double *__restrict__ n2b, const int *__restrict__ mr2, const int *__restrict__ src2,
     int ma2, int L, int nm1, int nm2, int rep_pad, double cy, const int2 *__restrict__ order,
     const WinArgs wa, double *__restrict__ out) {
-    // two row-operand buffers; the segment partial sums of the epilogue reuse the same
-    // memory once the Gram products are done
-    // (int8-split form: the two row buffers are two arrays, the first shared with P)
-    constexpr int OPS = NS ? NS * 4096 / 8 : 2 * TS * LDA, EPI = TS * PP;
+    // two row-operand buffers (int8-split form: and the waves' column regions); the
+    // segment partial sums of the epilogue reuse the same memory once the Gram products
+    // are done
+    constexpr int OPS = NS ? kI8Lds / 8 : 2 * TS * LDA, EPI = TS * PP;
     constexpr int NTM = NS ? 1 : 2;  // column tiles per row tile
     __shared__ double Sh[OPS > EPI ? OPS : EPI];
-    __shared__ double Sh2[NS ? NS * 4096 / 8 : 1];
     __shared__ int s_slot;
     double *const Abuf = Sh, *const P = Sh;
     __shared__ double n2r[TS], n2c[2][TS], mur[NS ? TS : 1], muc[NS ? TS : 1];
@@ -701,8 +703,8 @@ __global__ __launch_bounds__(NT, NS ? 2 : 3) void local_scalar_f64_kernel(
     ga.r16 = lane & 15;
     ga.kq = lane >> 4;
     GramI8Args g8;
-    g8.stk = (lane & 3) ^ ((lane >> 4) & 3);
-    g8.rdo = (lane & 15) * 64 + 16 * ((lane >> 4) ^ (((lane & 15) >> 2) & 3));
+    g8.stk = (lane & 3) ^ ((lane >> 3) & 3);
+    g8.rdo = (lane & 15) * 64 + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
     for (int k = tid; k < MB * AP; k += NT) accL[k] = 0.0;
     if (tid < 64) etab[tid] = kExp2Tab64[tid];  // ordered by the label loop's first barrier
 
@@ -814,7 +816,6 @@ __global__ __launch_bounds__(NT, NS ? 2 : 3) void local_scalar_f64_kernel(
                         gram_dispatch<1>(Abuf, ga, na16, ntw, c);
                 } else {
                     g8.pa = static_cast<const signed char *>(Xp1) + (size_t)(r0 + min(lrow, rn - 1)) * rep_pad;
-                    g8.pz = wa.zero2;
                     g8.pb = static_cast<const signed char *>(Xp2) + (size_t)(c0[0] + min(bcol, cn[0] - 1)) * rep_pad;
                     g8.w0 = w0;
                     g8.w1 = w1;
@@ -824,8 +825,7 @@ __global__ __launch_bounds__(NT, NS ? 2 : 3) void local_scalar_f64_kernel(
                     for (int x = 0; x < 4; ++x)
 #pragma unroll
                         for (int d = 0; d < NS; ++d) acc[x][d] = (v4i){0, 0, 0, 0};
-                    gram_i8_dispatch<NS>(reinterpret_cast<signed char *>(Sh),
-                                         reinterpret_cast<signed char *>(Sh2), g8, na16,
+                    gram_i8_dispatch<NS>(reinterpret_cast<signed char *>(Sh), g8, na16,
                                          wave * 16 < cn[0], acc);
                     // sum_d 2^-7d acc_d in FP64 (the common 2^-12 and the scales in the epilogue)
 #pragma unroll
@@ -957,8 +957,7 @@ int prepare_acc_slots(hipStream_t stream) {
 
 // Device copy of a plan plus the packed rows, their window masks and the row tiles'.
 // FP64 form (ns = 0): Xp rows of rep_pad doubles; int8-split form (ns slices): Xp rows of
-// rep_pad bytes (nwin windows of kI8WinBytes, pack_rows_i8) plus a zero row, and mu the
-// rows' scales.
+// rep_pad bytes (nwin windows of kI8WinBytes, pack_rows_i8) and mu the rows' scales.
 struct DevicePacked {
     void *Xp = nullptr;
     double *n2 = nullptr, *mu = nullptr;
@@ -1071,7 +1070,7 @@ int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const doub
     const size_t plan_n = mr_n + src_n + perm_n + nla + 1;
     char *plan = static_cast<char *>(workspace(ws_plan, sizeof(int) * plan_n, stream));
     // operand rows, then n2 (and mu) of the rows
-    const size_t op_bytes = ((ns ? (size_t)(p.rows + 1) * d.rep_pad
+    const size_t op_bytes = ((ns ? (size_t)p.rows * d.rep_pad
                                  : sizeof(double) * (size_t)p.rows * d.rep_pad) + 255) / 256 * 256;
     const size_t xp_bytes = op_bytes + sizeof(double) * (2 * (size_t)p.rows + 64);
     const size_t rm_off = (xp_bytes + 255) / 256 * 256;
@@ -1097,8 +1096,8 @@ int upload_and_pack(const LocalPlan &p, const std::vector<int> &perm, const doub
     d.rowmask = reinterpret_cast<unsigned long long *>(pack + rm_off);
     d.tmask = masked ? reinterpret_cast<ulonglong2 *>(pack + tm_off) : nullptr;
     if (p.rows > 0) {
-        if (ns == kI8Slices)  // (one more wave writes the zero row)
-            hipLaunchKernelGGL(pack_rows_i8<kI8Slices>, dim3((p.rows + 4) / 4), dim3(256), 0,
+        if (ns == kI8Slices)
+            hipLaunchKernelGGL(pack_rows_i8<kI8Slices>, dim3((p.rows + 3) / 4), dim3(256), 0,
                                stream, X, d.src, d.perm, p.max_atoms, p.rows, rep, d.nwin,
                                static_cast<signed char *>(d.Xp), d.n2, d.mu, d.rowmask);
         else
@@ -1125,8 +1124,6 @@ WinArgs win_args(const DevicePacked &d1, const DevicePacked &d2) {
     w.perm2 = d2.perm;
     w.mu1 = d1.mu;
     w.mu2 = d2.mu;
-    w.zero2 = d2.mu ? static_cast<const signed char *>(d2.Xp) + (size_t)d2.rows * d2.rep_pad
-                    : nullptr;
     w.nblk1 = d1.nblk;
     w.nblk2 = d2.nblk;
     w.nwin = d1.nwin;
