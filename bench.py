@@ -39,6 +39,9 @@ PEAK_BF16_MFMA_TFLOPS = 2500.0
 PEAK_INT8_MFMA_TOPS = 5000.0
 SIGMA, LAMBDA = 2.0, 1e-8
 ENERGY_REL_BAR = 1e-6
+# a reduced-precision path replaces the FP64 fit as the head only when it is clearly faster:
+# fit times repeat to about 1% between runs, and on a tie the reference precision stands
+HEAD_MARGIN = 0.98
 ELEMENTS = (1, 6, 7, 8, 16)
 
 
@@ -660,13 +663,13 @@ def main(argv=None):
                                                 np.max(np.abs(Kte_h))),
                 speedup_vs_fp64=res["fp64"][0] / res["mixed"][0])
     # The metric is the fit time at <= 1e-6 relative energy error vs FP64: a reduced-
-    # precision path is the head only when its test energies meet that bar (and it is
-    # faster than every other path that does).
+    # precision path is the head only when its test energies meet that bar and it is
+    # faster than every other path that does, by more than the run-to-run spread.
     head = "fp64" if "fp64" in res else ("i8x6" if "i8x6" in res else "mixed")
     if "fp64" in res:
         for cand, acc in (("i8x6", acc_i8), ("mixed", accuracy)):
             if (cand in res and acc.get("energy_rel_err_max", 1.0) <= ENERGY_REL_BAR
-                    and res[cand][0] < res[head][0]):
+                    and res[cand][0] < HEAD_MARGIN * res[head][0]):
                 head = cand
     if accuracy:
         accuracy["meets_1e-6_bar"] = bool(accuracy.get("energy_rel_err_max", 1.0) <= ENERGY_REL_BAR)
