@@ -459,10 +459,14 @@ struct GramI8Args {
 constexpr int kVmcnt0 = 0x0F70;
 constexpr int kLgkmcnt0 = 0xC07F;
 
-// Window of active entry 4c + k (c uniform across the wave, k per lane).
+// Window of active entry 4c + k (c uniform across the wave, k = 0..3 per lane): the
+// chunk's four windows are read as uniform values and picked per lane (no LDS round trip).
 __device__ __forceinline__ int win_i8(const GramI8Args &g, int c, int k) {
-    const int e = 4 * c + k;
-    return c >= kMaxWin / 4 ? e : __shfl(c < 16 ? g.w0 : g.w1, e & 63);
+    if (c >= kMaxWin / 4) return 4 * c + k;
+    const int src = c < 16 ? g.w0 : g.w1, base = (4 * c) & 63;
+    const int v0 = __builtin_amdgcn_readlane(src, base), v1 = __builtin_amdgcn_readlane(src, base + 1);
+    const int v2 = __builtin_amdgcn_readlane(src, base + 2), v3 = __builtin_amdgcn_readlane(src, base + 3);
+    return k < 2 ? (k == 0 ? v0 : v1) : (k == 2 ? v2 : v3);
 }
 
 // Row chunk c into the row buffer buf: wave w fills bytes [q 4096 + w 1024, +1024) of
