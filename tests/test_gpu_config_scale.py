@@ -4,7 +4,8 @@ random or small inputs):
 * 1,000 molecules (seed 0), 800 training: alpha of the RFP Cholesky solve at
   lambda = 1e-8 against the oracle's cho_solve_rfp, on the same kernel and end to end.
 * 20,000 molecules (seed 1): descriptors of a 64-molecule sample, a 64 x 20,000 strip of
-  the FP64 RFP kernel and of the BF16x3 kernel against the oracle, and the FP64 RFP
+  the FP64 RFP kernel, of the int8-split (i8x6) kernel and of the BF16x3 kernel against
+  the oracle, and the FP64 RFP
   Cholesky at lambda = 1e-8 judged by its backward error, with the residual taken
   through kf_rfp_symv on a kept copy of the kernel.
 * the learned-PCG pieces at the 2,000-conformer workload's lambda = 1e-10 (8 conformers
@@ -117,6 +118,27 @@ def test_config1_bf16x3_kernel_strip(config1):
     err = rel_err(got, c["strip"])
     print(f"20k BF16x3 kernel strip (64 x 20,000) vs oracle: {err:.2e}")
     assert err < 2e-5
+
+
+def test_config1_i8x6_kernel_strip(config1):
+    """The FP64-accurate int8-split form at the workload's size: every entry of the strip
+    within 1e-10 relative of the oracle (the form's error on FCHL19 descriptors is ~1e-11)."""
+    from kernelforge_amd import capi
+
+    c = config1
+    nm = c["nm"]
+    arf8 = capi.DeviceArray(nm * (nm + 1) // 2)
+    capi.local_kernel_symm_rfp(capi.KF_PREC_I8X6, c["X"].ptr, c["dQ"].ptr, c["dN"].ptr, nm,
+                               c["ma"], c["rs"], 2.0, arf8.ptr)
+    capi.synchronize()
+    got = _strip_from_rfp(arf8.numpy(), nm, c["sample"])
+    ref = c["strip"]
+    big = np.abs(ref) > 1e-200
+    entry = float(np.max(np.abs(got - ref)[big] / np.abs(ref)[big]))
+    print(f"20k i8x6 kernel strip (64 x 20,000) vs oracle: max entry rel {entry:.2e}, "
+          f"normwise {rel_err(got, ref):.2e}")
+    assert entry < 1e-10
+    assert np.all(np.abs(got - ref)[~big] <= 1e-200)
 
 
 def test_config1_fp64_cholesky_backward_error(config1):
