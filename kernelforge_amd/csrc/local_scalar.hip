@@ -23,6 +23,12 @@
 //    molecule, then the row molecules' sums into the block's accumulators.  Deterministic
 //    (no atomics on values); each K entry is written exactly once, straight into RFP
 //    (UPLO='U') or dense layout.
+//  * The same kernel template also runs an FP64-accurate int8-split form (precision
+//    KF_PREC_I8X6, template slice count 6): each packed row is stored as six 7-bit int8
+//    slices under a power-of-two scale (pack_rows_i8), the Gram products run exactly in
+//    int32 on v_mfma_i32_16x16x64_i8 (one column tile per row tile, both operands staged
+//    through LDS with LDS-destination loads), and the FP64 epilogue above receives the
+//    recombined Gram entries (gram_i8_tile).
 #include <algorithm>
 #include <mutex>
 #include <cmath>
