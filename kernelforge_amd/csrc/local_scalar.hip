@@ -168,11 +168,12 @@ __global__ __launch_bounds__(256) void pack_rows_f64(const double *__restrict__ 
 // each integer slice q_s in [-64, 64] the rounded remainder of the ones before it (every
 // step is exact in FP64).  A Gram entry is then the exact integer dot products of slice
 // pairs (s, t) with s + t < NS, summed over each anti-diagonal d = s + t in int32 on the
-// MFMA and combined in FP64 as mu_x mu_y sum_d 2^-(12 + 7 d) acc_d.  A dropped pair
-// (s + t >= NS) weighs at most 2^-7(s+t) mu_x mu_y per summed column and the remainder r
-// of a row after NS slices at most 2^-(7 NS + 1) mu, so with NS = 6 a Gram entry is off by
-// at most 6 * 2^-42 mu_x mu_y per column (in practice far less: kernel entries within
-// ~1e-11 relative on FCHL19 descriptors).
+// MFMA and combined in FP64 as mu_x mu_y sum_d 2^-(12 + 7 d) acc_d.  Slice s is at most
+// 2^-7s mu in magnitude, so a dropped pair (s + t >= NS) weighs at most 2^-7(s+t) mu_x mu_y
+// per summed column (with NS = 6: five pairs on d = 6, 5.03 * 2^-42 over all of them), and
+// the remainders r (half a unit of the last slice, 2^-42) add at most 2 * 2^-42 mu_x mu_y:
+// a Gram entry is off by less than 8 * 2^-42 mu_x mu_y per column (in practice far less:
+// kernel entries within ~1e-11 relative on FCHL19 descriptors).
 // Layout: rows of nwin4 windows (16 columns each, nwin4 a multiple of 4, the columns past
 // rep zero), each window its NS slices of 16 bytes (padding windows to 128-byte lines
 // fetched more, not less).

@@ -6,11 +6,12 @@ whose weights sum past 2^-42 are dropped, and norms, exp and sums are FP64.
 Bars:
 * rows whose bits fit the kept slice pairs: Gram entries exact, so the kernel must match
   the oracle to the exact-argument exp tolerance of the FP64 form;
-* random and windowed sets: entry by entry within the method's bound.  A dropped pair
-  (s, t), s + t >= 6, weighs at most 2^-(7 (s+t)) mu_i mu_j per column, and the remainder
-  of each row after six slices at most 2^-43 mu, so |dG_ij| <= 6 * 2^-42 mu_i mu_j per
-  summed column, and each exp term moves by at most its value times 2 |dG_ij| / 2 sigma^2
-  (plus the FP64 form's rounding).  A dropped anti-diagonal, a wrong slice weight or a
+* random and windowed sets: entry by entry within the method's bound.  Slice s is at most
+  2^-7s mu, so a dropped pair (s, t), s + t >= 6, weighs at most 2^-(7 (s+t)) mu_i mu_j
+  per column (5.03 * 2^-42 over all of them), and the remainder of each row after six
+  slices is at most 2^-42 mu (half a unit of the last slice), adding 2 * 2^-42 mu_i mu_j:
+  |dG_ij| < 8 * 2^-42 mu_i mu_j per summed column, and each exp term moves by at most its
+  value times expm1(2 |dG_ij| / 2 sigma^2) (plus the FP64 form's rounding).  A dropped anti-diagonal, a wrong slice weight or a
   wrong row scale breaks the bound by orders of magnitude;
 * FCHL19 descriptors (config-0 molecules, sigma 2): entries within 1e-10 relative;
 * the energy fit of the 800/200 config-0 split at lambda = 1e-8: within 1e-6 relative of
@@ -56,7 +57,7 @@ def i8_bound(x1, q1, n1, x2, q2, n2, sigma):
     d2 = (R1 * R1).sum(1)[:, None] + (R2 * R2).sum(1)[None, :] - 2.0 * R1 @ R2.T
     c = 1.0 / (2.0 * sigma * sigma)
     term = np.exp(-np.maximum(d2, 0.0) * c) * (L1[:, None] == L2[None, :])
-    dg = 6.0 * 2.0**-42 * mu(R1)[:, None] * mu(R2)[None, :] * x1.shape[2]
+    dg = 8.0 * 2.0**-42 * mu(R1)[:, None] * mu(R2)[None, :] * x1.shape[2]
     b = term * np.expm1(2.0 * dg * c)
     out = np.zeros((len(n1), len(n2)))
     np.add.at(out, (np.repeat(M1, len(M2)), np.tile(M2, len(M1))), b.ravel())
