@@ -292,7 +292,7 @@ def learned_pcg_leg(args, L):
     the shift network is trained on kernel systems of random 250-conformer subsets, then
     PCG with the matrix-free full matvec runs with the learned block-Jacobi correction and
     with plain block-Jacobi (fallback), both capped; FP64 RFP Cholesky on the full system
-    is the direct solve and the judge of the error."""
+    is the direct solve and the reference the PCG energies are measured against."""
     from kernelforge_amd import capi, synthetic
     from kernelforge_amd import learned_pcg as LP
 
@@ -344,7 +344,7 @@ def learned_pcg_leg(args, L):
                      "history_every_10": [float(v) for v in h[::10]]}
         log(f"learned PCG: {name}: {it} iterations, true residual {rel:.3e}, {dt:.2f} s "
             f"({dt / max(it, 1) * 1e3:.1f} ms/iteration)")
-    # ---- direct FP64 RFP Cholesky on the full system (the judge) ----
+    # ---- direct FP64 RFP Cholesky on the full system (the reference solution) ----
     alpha = capi.DeviceArray(full.n)
     t0 = time.perf_counter()
     full.build_rfp()
@@ -481,8 +481,9 @@ def main(argv=None):
                          "mixed; all: the three, each reported with its error vs the FP64 fit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the path would shard by molecule-pair tiles, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the kernel "
+              "would shard by blocks of molecule pairs, one block range per GPU, with the "
+              "Cholesky distributed by block columns)", file=sys.stderr)
         return 2
 
     from kernelforge_amd import capi, synthetic
