@@ -154,16 +154,21 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
     __syncthreads();
 }
 
-// POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB], one wave.
-// Lane i keeps row i of the tile in registers.  Cholesky, right-looking: per column j
-// the pivot comes from lane j (readlane), every lane scales its L[i][j] and publishes
-// it in LDS, and updates its row with the column read back (broadcast reads; one
-// wave, so LDS order needs no barrier).  Inverse, blocked on the two 32-row halves,
+// POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB], four waves.
+// Thread (wave q, lane i) keeps columns 16q .. 16q+15 of row i in registers.  Cholesky,
+// right-looking, one barrier per column j: the wave that owns column j takes the pivot
+// from lane j (readlane), scales its L[i][j] and publishes the column in LDS; after the
+// barrier every wave that owns columns right of j updates them with the column read
+// back (broadcast reads).  Columns alternate between two LDS buffers (a buffer is
+// rewritten two steps later, after every wave has passed the barrier that follows its
+// reads), and so does the failure flag, so all waves leave at the same barrier.  The
+// column loop runs as four rolled quarters of 16 unrolled steps: every register index is
+// a compile-time constant.  Inverse, blocked on the two 32-row halves,
 //   L = [[L0, 0], [B, L1]],  L^-1 = [[L0^-1, 0], [-L1^-1 B L0^-1, L1^-1]]:
-// lane i runs forward substitution for column i % 32 of the inverse of diagonal block
-// i / 32 (both blocks at once, 32 steps), and the off-diagonal block is two 32x32x32
-// MFMA products.  A partial tile (w < 64) is padded with the identity, which keeps both
-// factors block diagonal.
+// lane i of wave 0 runs forward substitution for column i % 32 of the inverse of diagonal
+// block i / 32 (both blocks at once, 32 steps) while waves 1..3 store the factor, and the
+// off-diagonal block is two 32x32x32 MFMA products, one 16x16 fragment per wave.  A partial
+// tile (w < 64) is padded with the identity, which keeps both factors block diagonal.
 template <typename T>
 __device__ __forceinline__ T lane_value(T v, int l);
 template <>
@@ -183,83 +188,104 @@ constexpr int HB = NB / 2;   // half block of the blocked inverse
 constexpr int HP = HB + 1;   // LDS pitch of its pieces
 
 template <typename T>
-__global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
+__global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
     __shared__ T S[NB * TP];
-    __shared__ T col[NB];
+    __shared__ T col[2][NB];
     __shared__ T dinv[NB];
     __shared__ T Di[NB * HP];  // row 32h + q, column c: (L_h^-1)[q][c]
     __shared__ T Zs[HB * HP];  // B L0^-1
+    __shared__ int fail_s[2];
     if (*info) return;
-    // this wave is the whole critical path of the factorization: let it issue ahead of the
-    // trailing-update waves that share its CU
+    // these waves are the whole critical path of the factorization: let them issue ahead of
+    // the trailing-update waves that share their CU
     __builtin_amdgcn_s_setprio(3);
-    const int i = threadIdx.x;
+    const int tid = threadIdx.x, i = tid & 63, q = tid >> 6;
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
-    load_tile<T, true, 64>(a, t, w, w, S);
+    load_tile<T, true, 256>(a, t, w, w, S);
+    if (tid < 2) fail_s[tid] = 0;
     __syncthreads();
-    if (i >= w) S[i * TP + i] = T(1);
+    if (tid >= w && tid < NB) S[tid * TP + tid] = T(1);
     __syncthreads();
-    T r[NB];
+    T r[16];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) r[c] = c <= i ? S[i * TP + c] : T(0);
+    for (int c = 0; c < 16; ++c) r[c] = 16 * q + c <= i ? S[i * TP + 16 * q + c] : T(0);
     int fail = 0;
+    for (int jb = 0; jb < 4 && !fail; ++jb) {
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const T piv = lane_value<T>(r[j], j);
-        if (!(piv > T(0))) {  // uniform: also catches NaN
-            fail = j + 1;
-            break;
+        for (int cj = 0; cj < 16; ++cj) {
+            const int j = 16 * jb + cj;
+            T *const cb = col[cj & 1];
+            if (q == jb) {  // the wave that owns column j (uniform)
+                const T piv = lane_value<T>(r[cj], j);
+                if (piv > T(0)) {
+                    const T rd = recip_sqrt(piv);
+                    r[cj] = i == j ? piv * rd : r[cj] * rd;
+                    cb[i] = r[cj];
+                    if (i == 0) dinv[j] = rd;
+                } else if (i == 0) {
+                    fail_s[cj & 1] = j + 1;  // also catches NaN
+                }
+            }
+            __syncthreads();
+            fail = fail_s[cj & 1];
+            if (fail) break;
+            if (q >= jb) {  // waves that own columns right of j
+                const T lij = cb[i];
+                T cv[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) cv[c] = cb[16 * q + c];
+                __builtin_amdgcn_sched_barrier(0);
+                if (q > jb) {
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) r[c] -= lij * cv[c];
+                } else {
+#pragma unroll
+                    for (int c = cj + 1; c < 16; ++c) r[c] -= lij * cv[c];
+                }
+            }
         }
-        const T rd = recip_sqrt(piv);
-        r[j] = i == j ? piv * rd : r[j] * rd;
-        col[i] = r[j];  // read back below by the same wave: LDS keeps its order
-        if (i == 0) dinv[j] = rd;
-        // all of column j first, then the updates: one wait for the whole batch
-        T cv[NB];
-#pragma unroll
-        for (int k = j + 1; k < NB; ++k) cv[k] = col[k];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = j + 1; k < NB; ++k) r[k] -= r[j] * cv[k];
     }
     if (fail) {
-        if (i == 0 && fail <= w) atomicCAS(info, 0, s0 + fail);
+        if (tid == 0 && fail <= w) atomicCAS(info, 0, s0 + fail);
         return;
     }
 #pragma unroll
-    for (int c = 0; c < NB; ++c) S[i * TP + c] = c <= i ? r[c] : T(0);
+    for (int c = 0; c < 16; ++c) S[i * TP + 16 * q + c] = 16 * q + c <= i ? r[c] : T(0);
     __syncthreads();
-    store_tile<T, true>(a, t, w, w, S);
-    // column cc of L_h^-1 (h = i / 32): x[q] = (delta_q,cc - sum_{k<q} L_h[q][k] x[k]) / L_h[q][q],
-    // with the reciprocals of the diagonal kept from the factorization
     const int h = i >> 5, cc = i & (HB - 1);
-    {
+    if (q == 0) {
+        // column cc of L_h^-1 (h = i / 32): x[p] = (delta_p,cc - sum_{k<p} L_h[p][k] x[k]) / L_h[p][p],
+        // with the reciprocals of the diagonal kept from the factorization
         T x[HB];
 #pragma unroll
-        for (int q = 0; q < HB; ++q) {
-            const int g = h * HB + q;
+        for (int p = 0; p < HB; ++p) {
+            const int g = h * HB + p;
             T lq[HB];
 #pragma unroll
-            for (int k = 0; k < q; ++k) lq[k] = S[g * TP + h * HB + k];
+            for (int k = 0; k < p; ++k) lq[k] = S[g * TP + h * HB + k];
             const T dq = dinv[g];
             __builtin_amdgcn_sched_barrier(0);
-            T acc[4] = {q == cc ? T(1) : T(0), T(0), T(0), T(0)};
+            T acc[4] = {p == cc ? T(1) : T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int k = 0; k < q; ++k) acc[k & 3] -= lq[k] * x[k];
-            x[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * dq;
+            for (int k = 0; k < p; ++k) acc[k & 3] -= lq[k] * x[k];
+            x[p] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * dq;
         }
 #pragma unroll
-        for (int q = 0; q < HB; ++q) Di[(h * HB + q) * HP + cc] = x[q];
+        for (int p = 0; p < HB; ++p) Di[(h * HB + p) * HP + cc] = x[p];
+    } else {  // waves 1..3 store the factor meanwhile (coalesced along the contiguous axis)
+        for (int idx = tid - 64; idx < NB * NB; idx += 192) {
+            int rr, c;
+            if (t.rs == 1) { rr = idx % NB; c = idx / NB; } else { c = idx % NB; rr = idx / NB; }
+            if (rr < w && c < w && rr >= c) a[t.base + rr * t.rs + c * t.cs] = S[rr * TP + c];
+        }
     }
+    __syncthreads();
     // Z = B L0^-1, then W = -L1^-1 Z, as 16x16x4 MFMA fragments (A[m][k] from lane
-    // m + 16k, B[k][n] from lane n + 16k), one 16x16 fragment at a time (the Cholesky above
-    // holds the register file); one wave, so the LDS order holds
+    // m + 16k, B[k][n] from lane n + 16k), fragment q on wave q
     typedef typename Vec4<T>::type V4;
-    const int l16 = i & 15, kq = i >> 4;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const int u = f >> 1, v = f & 1;
+    const int l16 = i & 15, kq = i >> 4, u = q >> 1, v = q & 1;
+    {
         V4 z = V4{0, 0, 0, 0};
 #pragma unroll
         for (int k0 = 0; k0 < HB; k0 += 4)
@@ -267,25 +293,25 @@ __global__ __launch_bounds__(64) void potrf_diag(T *a, Part P, int J, T *inv, in
 #pragma unroll
         for (int e = 0; e < 4; ++e) Zs[(u * 16 + mfma16_row<T>(i, e)) * HP + v * 16 + l16] = z[e];
     }
-    // inv (row-major, NB x NB): the lower-left block W, the diagonal blocks from Di, zeros above
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const int u = f >> 1, v = f & 1;
+    __syncthreads();
+    {
         V4 z = V4{0, 0, 0, 0};
 #pragma unroll
         for (int k0 = 0; k0 < HB; k0 += 4)
             z = mfma4(-Di[(HB + u * 16 + l16) * HP + k0 + kq], Zs[(k0 + kq) * HP + v * 16 + l16], z);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int q = HB + u * 16 + mfma16_row<T>(i, e), c = v * 16 + l16;
-            inv[q * NB + c] = (q < w && c < w) ? z[e] : T(0);
+            const int rr = HB + u * 16 + mfma16_row<T>(i, e), c = v * 16 + l16;
+            inv[rr * NB + c] = (rr < w && c < w) ? z[e] : T(0);
         }
     }
+    // inv (row-major, NB x NB): the diagonal blocks from Di, zeros above them; rows 16q ..
+    // 16q+15 on wave q
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-        const int hq = q / HB;
-        if (hq == h) inv[q * NB + i] = (q < w && i < w) ? Di[q * HP + cc] : T(0);
-        else if (hq == 0) inv[q * NB + i] = T(0);  // above the diagonal blocks
+    for (int p = 16 * q; p < 16 * q + 16; ++p) {
+        const int hq = p / HB;
+        if (hq == h) inv[p * NB + i] = (p < w && i < w) ? Di[p * HP + cc] : T(0);
+        else if (hq == 0) inv[p * NB + i] = T(0);  // above the diagonal blocks
     }
 }
 
@@ -1061,7 +1087,7 @@ int potrf_impl(T *a, int n, double l2, hipStream_t stream) {
             if (J > J0)
                 hipLaunchKernelGGL(syrk_range<T>, dim3((unsigned)range_tiles(P, J, J + 1)),
                                    dim3(256), 0, s2, a, P, J0, J, J, J + 1, info);
-            hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s2, a, P, J, inv, info);
+            hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, s2, a, P, J, inv, info);
             const int m = P.nblk - J - 1;
             if (m > 0)
                 hipLaunchKernelGGL(trsm_panel<T>, dim3(m), dim3(256), 0, s2, a, P, J, inv, info);
