@@ -158,10 +158,10 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
 // Thread (wave q, lane i) keeps columns 16q .. 16q+15 of row i in registers.  Cholesky,
 // right-looking, one barrier per column j: the wave that owns column j takes the pivot
 // from lane j (readlane), scales its L[i][j] and publishes the column in LDS; after the
-// barrier every wave that owns columns right of j updates them with the column read
-// back (broadcast reads).  Columns alternate between two LDS buffers (a buffer is
-// rewritten two steps later, after every wave has passed the barrier that follows its
-// reads), and so does the failure flag, so all waves leave at the same barrier.  The
+// barrier every wave updates its columns with the column read back (broadcast reads).
+// Columns and their pivots alternate between two LDS buffers (a buffer is rewritten two
+// steps later, after every wave has passed the barrier that follows its reads), and
+// every wave tests the same pivot, so all waves leave at the same barrier.  The
 // column loop runs as four rolled quarters of 16 unrolled steps: every register index is
 // a compile-time constant.  Inverse, blocked on the two 32-row halves,
 //   L = [[L0, 0], [B, L1]],  L^-1 = [[L0^-1, 0], [-L1^-1 B L0^-1, L1^-1]]:
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
     __shared__ T dinv[NB];
     __shared__ T Di[NB * HP];  // row 32h + q, column c: (L_h^-1)[q][c]
     __shared__ T Zs[HB * HP];  // B L0^-1
-    __shared__ int fail_s[2];
+    __shared__ T pv[2];  // the pivots of the last two columns
     if (*info) return;
     // these waves are the whole critical path of the factorization: let them issue ahead of
     // the trailing-update waves that share their CU
@@ -203,47 +203,47 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
     const int s0 = P.start(J), w = P.width(J);
     const Tile t = tile_at(P, s0, s0);
     load_tile<T, true, 256>(a, t, w, w, S);
-    if (tid < 2) fail_s[tid] = 0;
     __syncthreads();
     if (tid >= w && tid < NB) S[tid * TP + tid] = T(1);
     __syncthreads();
     T r[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) r[c] = 16 * q + c <= i ? S[i * TP + 16 * q + c] : T(0);
+    // The published column holds zeros above the diagonal, so every wave applies the same
+    // 16 updates each step: columns left of j meet those zeros (exact no-ops) and the owner
+    // restores its column j afterwards.  The pivot travels with the column, so every wave
+    // sees a failure (a pivot that is not positive, or NaN) after the same barrier.
     int fail = 0;
     for (int jb = 0; jb < 4 && !fail; ++jb) {
 #pragma unroll
         for (int cj = 0; cj < 16; ++cj) {
+            if (fail) continue;  // uniform: every wave skips the remaining steps together
             const int j = 16 * jb + cj;
             T *const cb = col[cj & 1];
             if (q == jb) {  // the wave that owns column j (uniform)
                 const T piv = lane_value<T>(r[cj], j);
-                if (piv > T(0)) {
-                    const T rd = recip_sqrt(piv);
-                    r[cj] = i == j ? piv * rd : r[cj] * rd;
-                    cb[i] = r[cj];
-                    if (i == 0) dinv[j] = rd;
-                } else if (i == 0) {
-                    fail_s[cj & 1] = j + 1;  // also catches NaN
+                const T rd = recip_sqrt(piv);  // unused when piv is not positive
+                const T l = i == j ? piv * rd : r[cj] * rd;
+                cb[i] = i >= j ? l : T(0);
+                if (i == 0) {
+                    pv[cj & 1] = piv;
+                    dinv[j] = rd;
                 }
             }
             __syncthreads();
-            fail = fail_s[cj & 1];
-            if (fail) break;
-            if (q >= jb) {  // waves that own columns right of j
-                const T lij = cb[i];
-                T cv[16];
+            const T piv = pv[cj & 1];
+            const T lij = cb[i];
+            T cv[16];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) cv[c] = cb[16 * q + c];
-                __builtin_amdgcn_sched_barrier(0);
-                if (q > jb) {
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) r[c] -= lij * cv[c];
-                } else {
-#pragma unroll
-                    for (int c = cj + 1; c < 16; ++c) r[c] -= lij * cv[c];
-                }
+            for (int c = 0; c < 16; ++c) cv[c] = cb[16 * q + c];
+            __builtin_amdgcn_sched_barrier(0);
+            if (!(piv > T(0))) {
+                fail = j + 1;
+                continue;
             }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) r[c] -= lij * cv[c];
+            if (q == jb) r[cj] = lij;
         }
     }
     if (fail) {
