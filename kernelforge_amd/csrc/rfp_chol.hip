@@ -156,14 +156,10 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
 
 // POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB], four waves.
 // Thread (wave q, lane i) keeps columns 16q .. 16q+15 of row i in registers.  Cholesky,
-// right-looking, one barrier per column j: the wave that owns column j takes the pivot
-// from lane j (readlane), scales its L[i][j] and publishes the column in LDS; after the
-// barrier every wave updates its columns with the column read back (broadcast reads).
-// Columns and their pivots alternate between two LDS buffers (a buffer is rewritten two
-// steps later, after every wave has passed the barrier that follows its reads), and
-// every wave tests the same pivot, so all waves leave at the same barrier.  The
-// column loop runs as four rolled quarters of 16 unrolled steps: every register index is
-// a compile-time constant.  Inverse, blocked on the two 32-row halves,
+// blocked right-looking over the four 16-column quarters: the wave that owns a quarter
+// factors its 64 x 16 panel alone (pivots and in-panel updates through readlane, no
+// barrier) and publishes it in LDS; after one barrier every wave right of it applies the
+// rank-16 update to its own columns.  Inverse, blocked on the two 32-row halves,
 //   L = [[L0, 0], [B, L1]],  L^-1 = [[L0^-1, 0], [-L1^-1 B L0^-1, L1^-1]]:
 // lane i of wave 0 runs forward substitution for column i % 32 of the inverse of diagonal
 // block i / 32 (both blocks at once, 32 steps) while waves 1..3 store the factor, and the
@@ -189,12 +185,14 @@ constexpr int HP = HB + 1;   // LDS pitch of its pieces
 
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, int *info) {
+    constexpr int PQ = 18;  // pitch of a published quarter panel (16-byte aligned rows)
+    constexpr int WK = 2 * NB * PQ > (NB + HB) * HP ? 2 * NB * PQ : (NB + HB) * HP;
     __shared__ T S[NB * TP];
-    __shared__ T col[2][NB];
+    __shared__ T Wk[WK];  // the quarter panels while factoring, then Di and Zs
     __shared__ T dinv[NB];
-    __shared__ T Di[NB * HP];  // row 32h + q, column c: (L_h^-1)[q][c]
-    __shared__ T Zs[HB * HP];  // B L0^-1
-    __shared__ T pv[2];  // the pivots of the last two columns
+    __shared__ int fail_s[2];
+    T *const Di = Wk;            // row 32h + q, column c: (L_h^-1)[q][c]
+    T *const Zs = Wk + NB * HP;  // B L0^-1
     if (*info) return;
     // these waves are the whole critical path of the factorization: let them issue ahead of
     // the trailing-update waves that share their CU
@@ -209,41 +207,46 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
     T r[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) r[c] = 16 * q + c <= i ? S[i * TP + 16 * q + c] : T(0);
-    // The published column holds zeros above the diagonal, so every wave applies the same
-    // 16 updates each step: columns left of j meet those zeros (exact no-ops) and the owner
-    // restores its column j afterwards.  The pivot travels with the column, so every wave
-    // sees a failure (a pivot that is not positive, or NaN) after the same barrier.
+    // Quarter jb: its wave factors the 64 x 16 panel alone (pivots and in-panel updates by
+    // readlane, no barrier), zeroing the entries above the diagonal, and publishes it; after
+    // one barrier every wave right of it applies the rank-16 update to its own columns.  The
+    // panel and the failure flag alternate between two buffers (rewritten two quarters
+    // later, after every wave has passed the barrier that follows its reads).
     int fail = 0;
-    for (int jb = 0; jb < 4 && !fail; ++jb) {
+    for (int jb = 0; jb < 4; ++jb) {
+        T *const pq = Wk + (jb & 1) * NB * PQ;
+        if (q == jb) {  // uniform
+            int f = 0;
 #pragma unroll
-        for (int cj = 0; cj < 16; ++cj) {
-            if (fail) continue;  // uniform: every wave skips the remaining steps together
-            const int j = 16 * jb + cj;
-            T *const cb = col[cj & 1];
-            if (q == jb) {  // the wave that owns column j (uniform)
+            for (int cj = 0; cj < 16; ++cj) {
+                const int j = 16 * jb + cj;
                 const T piv = lane_value<T>(r[cj], j);
-                const T rd = recip_sqrt(piv);  // unused when piv is not positive
-                const T l = i == j ? piv * rd : r[cj] * rd;
-                cb[i] = i >= j ? l : T(0);
-                if (i == 0) {
-                    pv[cj & 1] = piv;
-                    dinv[j] = rd;
-                }
-            }
-            __syncthreads();
-            const T piv = pv[cj & 1];
-            const T lij = cb[i];
-            T cv[16];
+                if (!f && !(piv > T(0))) f = j + 1;  // also catches NaN; later values unused
+                const T rd = recip_sqrt(piv);
+                r[cj] = i == j ? piv * rd : (i > j ? r[cj] * rd : T(0));
+                if (i == 0) dinv[j] = rd;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) cv[c] = cb[16 * q + c];
-            __builtin_amdgcn_sched_barrier(0);
-            if (!(piv > T(0))) {
-                fail = j + 1;
-                continue;
+                for (int c = cj + 1; c < 16; ++c) r[c] -= r[cj] * lane_value<T>(r[cj], 16 * jb + c);
             }
 #pragma unroll
-            for (int c = 0; c < 16; ++c) r[c] -= lij * cv[c];
-            if (q == jb) r[cj] = lij;
+            for (int m = 0; m < 16; ++m) pq[i * PQ + m] = r[m];
+            if (i == 0) fail_s[jb & 1] = f;
+        }
+        __syncthreads();
+        fail = fail_s[jb & 1];
+        if (fail) break;
+        if (q > jb) {
+            T li[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) li[m] = pq[i * PQ + m];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const T *const pk = pq + (16 * q + c) * PQ;
+                T acc = r[c];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) acc -= li[m] * pk[m];
+                r[c] = acc;
+            }
         }
     }
     if (fail) {
