@@ -157,8 +157,8 @@ __device__ inline void tri_inverse(const T *L, U *Li, U *dgi, int w) {
 // POTRF of diagonal tile J plus its inverse (lower) into inv[NB*NB], four waves.
 // Thread (wave q, lane i) keeps columns 16q .. 16q+15 of row i in registers.  Cholesky,
 // blocked right-looking over the four 16-column quarters: the wave that owns a quarter
-// factors its 64 x 16 panel alone (pivots and in-panel updates through readlane, no
-// barrier) and publishes it in LDS; after one barrier every wave right of it applies the
+// factors its 64 x 16 panel alone (left-looking, the in-panel updates through readlane,
+// no barrier) and publishes it in LDS; after one barrier every wave right of it applies the
 // rank-16 update to its own columns.  Inverse, blocked on the two 32-row halves,
 //   L = [[L0, 0], [B, L1]],  L^-1 = [[L0^-1, 0], [-L1^-1 B L0^-1, L1^-1]]:
 // lane i of wave 0 runs forward substitution for column i % 32 of the inverse of diagonal
@@ -207,11 +207,12 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
     T r[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) r[c] = 16 * q + c <= i ? S[i * TP + 16 * q + c] : T(0);
-    // Quarter jb: its wave factors the 64 x 16 panel alone (pivots and in-panel updates by
-    // readlane, no barrier), zeroing the entries above the diagonal, and publishes it; after
-    // one barrier every wave right of it applies the rank-16 update to its own columns.  The
-    // panel and the failure flag alternate between two buffers (rewritten two quarters
-    // later, after every wave has passed the barrier that follows its reads).
+    // Quarter jb: its wave factors the 64 x 16 panel alone (left-looking, the in-panel
+    // updates by readlane, no barrier), zeroing the entries above the diagonal, and
+    // publishes it; after one barrier every wave right of it applies the rank-16 update to
+    // its own columns.  The panel and the failure flag alternate between two buffers
+    // (rewritten two quarters later, after every wave has passed the barrier that follows
+    // its reads).
     int fail = 0;
     for (int jb = 0; jb < 4; ++jb) {
         T *const pq = Wk + (jb & 1) * NB * PQ;
@@ -220,13 +221,19 @@ __global__ __launch_bounds__(256) void potrf_diag(T *a, Part P, int J, T *inv, i
 #pragma unroll
             for (int cj = 0; cj < 16; ++cj) {
                 const int j = 16 * jb + cj;
+                // left-looking inside the panel: column j first receives its finished
+                // columns' contributions (readlanes of final values, issued back to back)
+                if (cj > 0) {
+                    T acc[4] = {r[cj], T(0), T(0), T(0)};
+#pragma unroll
+                    for (int m = 0; m < cj; ++m) acc[m & 3] -= r[m] * lane_value<T>(r[m], j);
+                    r[cj] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+                }
                 const T piv = lane_value<T>(r[cj], j);
                 if (!f && !(piv > T(0))) f = j + 1;  // also catches NaN; later values unused
                 const T rd = recip_sqrt(piv);
                 r[cj] = i == j ? piv * rd : (i > j ? r[cj] * rd : T(0));
                 if (i == 0) dinv[j] = rd;
-#pragma unroll
-                for (int c = cj + 1; c < 16; ++c) r[c] -= r[cj] * lane_value<T>(r[cj], 16 * jb + c);
             }
 #pragma unroll
             for (int m = 0; m < 16; ++m) pq[i * PQ + m] = r[m];
