@@ -1006,15 +1006,29 @@ int molecule_windows(const double *X, const int *N, int nm, int ma, int rep, int
 }
 
 // Windows ranked rarest first by the number of molecules (of all given sets) using them.
+// The counts come from a histogram of byte values at each of the 16 byte positions of a
+// molecule's mask pair (16 increments a molecule instead of one per set bit).
 std::vector<int> rank_windows(const std::vector<const std::vector<unsigned long long> *> &sets,
                               int nwin) {
-    std::vector<long> freq(nwin, 0);
+    std::vector<long> hist(16 * 256, 0);
     for (const auto *m : sets)
-        for (size_t e = 0; e < m->size(); ++e)  // set bits only: word e & 1 holds windows 64 (e & 1) + ..
-            for (unsigned long long bits = (*m)[e]; bits; bits &= bits - 1) {
-                const int w = 64 * (int)(e & 1) + __builtin_ctzll(bits);
-                if (w < nwin) ++freq[w];
-            }
+        for (size_t e = 0; e < m->size(); ++e) {  // word e & 1 holds windows 64 (e & 1) + ..
+            const unsigned long long v = (*m)[e];
+            if (!v) continue;
+            const int h = (int)(e & 1);
+            for (int by = 0; by < 8; ++by) ++hist[256 * (8 * h + by) + ((v >> (8 * by)) & 255)];
+        }
+    std::vector<long> freq(nwin, 0);
+    for (int pos = 0; pos < 16; ++pos)
+        for (int v = 1; v < 256; ++v) {
+            const long c = hist[256 * pos + v];
+            if (!c) continue;
+            for (int b = 0; b < 8; ++b)
+                if ((v >> b) & 1) {
+                    const int w = 8 * pos + b;
+                    if (w < nwin) freq[w] += c;
+                }
+        }
     std::vector<int> rank(nwin);
     for (int w = 0; w < nwin; ++w) rank[w] = w;
     std::stable_sort(rank.begin(), rank.end(), [&](int x, int y) { return freq[x] < freq[y]; });
@@ -1023,27 +1037,54 @@ std::vector<int> rank_windows(const std::vector<const std::vector<unsigned long 
 
 // Molecule order: ascending by the bit string of the molecule's windows read rarest
 // first (the 64 rarest windows), ties in input order.  Molecules that use the same
-// rare windows end up in the same blocks, so the other blocks' tiles skip them.
+// rare windows end up in the same blocks, so the other blocks' tiles skip them.  A key is
+// 16 table lookups (the key bits of every byte value at each byte position of the mask
+// pair); the order is a stable LSD radix sort of the indices by key, 8 bits a pass,
+// skipping the passes whose byte is the same for every key (the same order as sorting
+// (key, index) pairs).
 std::vector<int> order_molecules(const std::vector<unsigned long long> &mask,
                                  const std::vector<int> &rank) {
+    typedef unsigned long long u64;
     const int nm = (int)(mask.size() / 2), nk = std::min<int>(64, (int)rank.size());
     // bit of window w in the key: rarest (rank position 0) most significant
     int shift[2 * 64];
     std::fill(shift, shift + 2 * 64, -1);
     for (int t = 0; t < nk; ++t) shift[rank[t]] = nk - 1 - t;
-    std::vector<std::pair<unsigned long long, int>> key(nm);
+    std::vector<u64> tab(16 * 256, 0ull);
+    for (int pos = 0; pos < 16; ++pos)
+        for (int v = 1; v < 256; ++v) {
+            u64 k = 0;
+            for (int b = 0; b < 8; ++b)
+                if ((v >> b) & 1) {
+                    const int sh = shift[8 * pos + b];
+                    if (sh >= 0) k |= 1ull << sh;
+                }
+            tab[256 * pos + v] = k;
+        }
+    std::vector<u64> key(nm);
     for (int a = 0; a < nm; ++a) {
-        unsigned long long k = 0;
-        for (int h = 0; h < 2; ++h)
-            for (unsigned long long bits = mask[2 * (size_t)a + h]; bits; bits &= bits - 1) {
-                const int sh = shift[64 * h + __builtin_ctzll(bits)];
-                if (sh >= 0) k |= 1ull << sh;
-            }
-        key[a] = {k, a};
+        u64 k = 0;
+        for (int h = 0; h < 2; ++h) {
+            const u64 m = mask[2 * (size_t)a + h];
+            if (!m) continue;
+            for (int by = 0; by < 8; ++by) k |= tab[256 * (8 * h + by) + ((m >> (8 * by)) & 255)];
+        }
+        key[a] = k;
     }
-    std::sort(key.begin(), key.end());  // (key, input index): ties stay in input order
-    std::vector<int> perm(nm);
-    for (int a = 0; a < nm; ++a) perm[a] = key[a].second;
+    std::vector<int> perm(nm), tmp(nm);
+    for (int a = 0; a < nm; ++a) perm[a] = a;
+    for (int pass = 0; pass < 8 && nm > 0; ++pass) {
+        const int sh = 8 * pass;
+        size_t cnt[257] = {0};
+        for (int a = 0; a < nm; ++a) ++cnt[((key[a] >> sh) & 255) + 1];
+        if (cnt[((key[0] >> sh) & 255) + 1] == (size_t)nm) continue;  // nothing moves
+        for (int v = 0; v < 256; ++v) cnt[v + 1] += cnt[v];
+        for (int a = 0; a < nm; ++a) {
+            const int m = perm[a];
+            tmp[cnt[(key[m] >> sh) & 255]++] = m;
+        }
+        perm.swap(tmp);
+    }
     return perm;
 }
 
