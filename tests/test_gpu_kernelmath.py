@@ -74,14 +74,18 @@ def test_cho_solve_rfp_wide_panels(km):
     assert np.linalg.norm(r) / np.linalg.norm(y) < 1e-12
 
 
-def test_not_spd_info_matches_lapack(km):
-    n = 70
+@pytest.mark.parametrize("n,col", [(70, 40), (70, 0), (70, 5), (70, 16), (70, 63), (70, 64),
+                                   (70, 69), (301, 149), (301, 150), (301, 200), (1001, 700)])
+def test_not_spd_info_matches_lapack(km, n, col):
+    """A negative diagonal entry at column col: the factorization stops there with LAPACK's
+    info (1-based column), whichever 16-column quarter of a 64-wide diagonal tile, partial
+    tile (70 and 301 end in one) or RFP region (the upper region starts at n / 2) it is in."""
     K = spd(n, 3)
-    K[40, 40] = -5.0
+    K[col, col] = -5.0
     arf = to_rfp(K)
     pftrf = sl.get_lapack_funcs("pftrf", (K,))
     _, info = pftrf(n, arf.copy(), transr="N", uplo="U")
-    assert info > 0
+    assert info == col + 1
     with pytest.raises(RuntimeError, match=f"DPFTRF failed, info={info}"):
         km.cho_solve_rfp(arf.copy(), np.ones(n), 0.0)
 
